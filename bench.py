@@ -1,0 +1,204 @@
+"""Benchmark of the fused IDWT + ICT back end on one MI355X.
+
+Workload (one step): decode one 4096x2160 3-component image whose
+coefficients sit in HBM -- 5-level irreversible 9/7 inverse DWT on float32
+subbands (LL ~ N(128,40), detail Laplace(4)*2^-level), inverse ICT, level
+shift, clamp, packed 24-bit BGR -- in one persistent launch of the fused
+engine, replayed from a hipGraph.  Prints one JSON line.
+
+Also measured, beside `value`: the plain reference-shaped GPU path (the
+drop-in BMI_comp_idwt per component + BMI_tile_ICT, 2 launches per level),
+the dominant kernel against HBM roofline, and the CPU oracle on the host.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import importlib
+import importlib.util
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+PKG_DIR = ROOT / "jpeg2000-decoder_amd"
+
+METRIC = "decoded Mpixel/s + IDWT HBM GB/s (% peak); served p50/p99 latency vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def load_package():
+    name = "jpeg2000_decoder_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(
+        name, PKG_DIR / "__init__.py", submodule_search_locations=[str(PKG_DIR)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(planes: np.ndarray, w: int, h: int, budget_s: float = 12.0):
+    """The CPU oracle (single-threaded scalar restatement of the reference
+    CPU back end) decoding the same image, repeated until ~budget_s."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle  # checker / CPU baseline only
+    oracle.lib()
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oracle.decode_tile(planes, levels=5, filt="97", out_format=2)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or reps >= 3:
+            break
+    mpix = reps * w * h / el / 1e6
+    return {"value": round(mpix, 3), "unit": "Mpixel/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} full 4096x2160x3 9/7 5-level IDWT+ICT decodes with the scalar C "
+                      f"oracle on 1 host core ({el:.1f} s)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--width", type=int, default=4096)
+    ap.add_argument("--height", type=int, default=2160)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--seg-rows", type=int, default=0)
+    ap.add_argument("--groups-per-cu", type=int, default=0)
+    ap.add_argument("--waves-per-group", type=int, default=0)
+    ap.add_argument("--persistent", type=int, default=1)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build decodes on a single MI355X "
+              "(tiles and images shard across GPUs with no exchange; see DESIGN.md). "
+              "Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import torch
+    pkg = load_package()
+    pkg.lib()
+    wl = importlib.import_module("jpeg2000_decoder_amd.workloads")
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    dev = torch.device("cuda:0")
+    W, H, L = args.width, args.height, 5
+
+    log("generating config: %dx%dx3 9/7 L=%d" % (W, H, L))
+    planes = wl.tile_planes(W, H, L, "97", 3, seed=0)
+    coef = [torch.from_numpy(planes[c].view(np.int32).copy()).to(dev) for c in range(3)]
+    out = torch.zeros((H, W * 3), dtype=torch.uint8, device=dev)
+    desc = pkg.tile_desc(coef_ptrs=[t.data_ptr() for t in coef], coef_pitch=W, width=W, height=H,
+                         levels=L, method=pkg.W9X7, ncomp=3, coef_kind=pkg.COEF_FLOAT,
+                         bit_depth=8, out_format=pkg.FMT_24BITS_RGB, out_ptr=out.data_ptr(),
+                         out_pitch_px=W)
+    tuning = pkg.BmiTuning(args.seg_rows, args.waves_per_group, args.groups_per_cu, args.persistent)
+    plan = pkg.BatchPlan([desc], tuning)
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+
+    def step():
+        if args.no_graph:
+            plan.execute(sh)
+        else:
+            plan.execute_graph(sh)
+
+    log(f"fused plan: {plan.num_items} work items, scratch {plan.scratch_bytes / 2**20:.1f} MiB")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    assert plan.error_flag() == 0, "fused kernel reported a dependency timeout"
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall0
+    ms_total = ev0.elapsed_time(ev1)
+    ms_step = ms_total / args.steps
+    mpix = W * H / (ms_step * 1e-3) / 1e6
+    log(f"fused: {ms_step * 1e3:.1f} us/step, {mpix:.0f} Mpixel/s (wall {t_wall:.3f} s)")
+
+    # algorithmic (compulsory) bytes per step: every coefficient read once
+    # (4 B), every output pixel written once (3 B for 24-bit BGR)
+    compulsory = W * H * 3 * 4 + W * H * 3
+    achieved = compulsory / (ms_step * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "fused_persistent_kernel", "algorithmic_bytes_per_launch": compulsory}
+
+    plain = None
+    if not args.no_plain:
+        log("plain path (BMI_comp_idwt x3 + BMI_tile_ICT)")
+        work = [torch.empty_like(t) for t in coef]
+        temp = torch.empty(W * H + 64, dtype=torch.int32, device=dev)
+        pdesc = pkg.tile_desc(coef_ptrs=[t.data_ptr() for t in work], coef_pitch=W, width=W,
+                              height=H, levels=L, method=pkg.W9X7, ncomp=3,
+                              coef_kind=pkg.COEF_FLOAT, bit_depth=8, out_format=pkg.FMT_24BITS_RGB,
+                              out_ptr=out.data_ptr(), out_pitch_px=W)
+        times = []
+        for it in range(args.warmup + args.steps):
+            for a, b in zip(work, coef):
+                a.copy_(b)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            pkg.plain_decode(pdesc, temp.data_ptr(), temp.numel() * 4, sh)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            if it >= args.warmup:
+                times.append(e0.elapsed_time(e1))
+        pms = float(np.mean(times))
+        plain = {"ms_per_step": round(pms, 4), "Mpixel_s": round(W * H / (pms * 1e-3) / 1e6, 1),
+                 "fused_speedup": round(pms / ms_step, 2),
+                 "launches_per_step": 3 * (2 * L) + 1}
+        log(f"plain: {pms * 1e3:.1f} us/step, fused speedup {pms / ms_step:.2f}x")
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle)")
+        cpu = cpu_baseline(planes, W, H)
+        log(f"cpu: {cpu['value']} Mpixel/s")
+
+    rec = {
+        "metric": METRIC,
+        "value": round(mpix, 1),
+        "unit": "Mpixel/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{W}x{H}x3 9/7 5-level IDWT + inverse ICT -> 24-bit BGR, "
+                               "coefficients resident in HBM, fused persistent launch "
+                               f"({'hipGraph' if not args.no_graph else 'direct'})",
+                   "work_items": plan.num_items},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_gpu": plain,
+    }
+    plan.close()
+    print(json.dumps(rec), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,196 @@
+/*
+ * bmi_gpulib.h -- C-ABI of the MI355X (gfx950) back end.
+ *
+ * Part 1 keeps, name for name and argument for argument, the entry points
+ * the reference host code calls into its GPU library
+ * (decoder/cudainterface/include/gpulib.h:24-57, implemented in
+ * decoder/cudainterface/src/gpulib.c:12-377), so CudaManager-style host
+ * code links against this library unchanged.  StreamId is hipStream_t.
+ *
+ * Compatible strengthening of the error convention: where the reference
+ * printed and called exit(-1) on any CUDA error (gpufunc.cu:9-22) or
+ * asserted on an unsupported mode, these functions never exit: int
+ * functions return a negative BMI_E* code, void functions record it, and
+ * BMI_last_error() returns the most recent code of the calling thread.
+ *
+ * Part 2 is additive (no counterpart in the reference): the fused /
+ * batched persistent decode, the plain-path driver used as its baseline,
+ * the request server and GPU T1.
+ */
+#ifndef BMI_GPULIB_H
+#define BMI_GPULIB_H
+
+#include "bmi_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define BMI_OK 0
+#define BMI_E_INVALID (-1)     /* bad argument / unsupported mode */
+#define BMI_E_NODEVICE (-2)    /* no HIP device */
+#define BMI_E_HIP (-3)         /* a HIP runtime call failed */
+#define BMI_E_NOMEM (-4)       /* allocation failed or over budget */
+#define BMI_E_UNSUPPORTED (-5) /* format/mode the reference also rejects */
+
+/* ------------------------------------------------------------------------
+ * Part 1: drop-in entry points
+ * --------------------------------------------------------------------- */
+
+/* gpulib.h:24 / gpufunc.cu:55-123: select device 0, fill name, CU count,
+   clock and memory; 0 or -1 when no device */
+int GpuInit(GpuInfo *gpfInfo);
+/* gpulib.h:25 / gpufunc.cu:125-131 */
+int GpuRemove(GpuInfo *gpfInfo);
+
+/* gpulib.h:27-28 / gpufunc.cu:134-187: by mType -- GPU_BUF linear
+   (mSize = mWidth*mHeight bytes when both are set), GPU_ALLIGN_BUF pitched
+   (mPitch filled, mSize = mPitch*mHeight), HOST_* pinned host memory */
+void MallocGpuBuf(CudaBuf *buf);
+void FreeGpuBuf(CudaBuf *buf);
+
+/* gpulib.h:30-33 / gpufunc.cu:189-217; GpuSync(NULL) syncs the device */
+StreamId CreateANewStream(void);
+void DestroyStream(StreamId *async_id);
+void GpuSync(StreamId *P_asyncId);
+
+/* gpulib.h:35-37 / gpulib.c:52-78: x offsets and widths in bytes, y in
+   rows; P_asyncId == NULL -> synchronous; *_no_pitch -> pitch = width.
+   GpuLinearMemCopy is asynchronous when async_id is not the null stream. */
+void Gpu2DCopy(CudaBuf *stripeBuf, CudaBuf *deviceBuf, int src_x_off, int src_y_off,
+               int dest_x_off, int dest_y_off, int width, int height, StreamId *P_asyncId,
+               CudaCopyDirection direction, int src_no_pitch, int dest_no_pitch);
+void Gpu2DCopyWithDiffPitch(CudaBuf *srcBuf, CudaBuf *destBuf, int src_x_off, int src_y_off,
+                            int dest_x_off, int dest_y_off, int src_pitch, int dest_pitch,
+                            int width, int height, StreamId *async_id,
+                            CudaCopyDirection direction, int src_no_pitch, int dest_no_pitch);
+void GpuLinearMemCopy(CudaBuf *srcBuf, CudaBuf *destBuf, int copy_size_in_byte,
+                      StreamId async_id, CudaCopyDirection direction);
+
+/* gpulib.h:44 / gpulib.c:165-185: multi-level IDWT of one tile-component,
+   in place in `orig` (Mallat layout at the tile offset, pitch
+   orig->mPitch); `temp` is transposed scratch (rows = tile columns, pitch
+   >= 4*tile height, starting at row tile->mOff.x, column tile->mOff.y).
+   toplevel -1 = all levels.  wordShift must be 2 (32-bit words).
+   5/3: two's-complement int32 in and out.  9/7: level-0 LL and every
+   detail band hold T1 sign-magnitude words, dequantised with
+   mfAbsStepNorm and the lifting gains; the output is float32. */
+int BMI_comp_idwt(TileInfo_c *tile, CompInfo_c *comp, CudaBuf *orig, CudaBuf *temp,
+                  StreamId asyncId, short wordShift, EncodeMathod mathod, int toplevel);
+/* gpulib.h:45 / gpulib.c:101-162: one synthesis level.  The reference
+   implements only 9/7 here (its 5/3 branch asserts, gpufunc.cu:602-612);
+   both filters are implemented. */
+int BMI_resolution_idwt(TileInfo_c *tile, CompInfo_c *comp, CudaBuf *orig, CudaBuf *temp,
+                        StreamId asyncId, short wordShift, EncodeMathod mathod,
+                        int resolutionLevel);
+
+/* gpulib.h:47-48 / gpulib.c:187-290: inverse RCT (5/3) / ICT (9/7) of
+   components 0..2 of the tile into tile->mResultBuf at the tile offset
+   (reduced to decodeLevel), packed per OutputFormat.  *_COMPONENTS and
+   NONE formats: MCT asserts in the reference, ICT returns; both return. */
+void BMI_tile_MCT(TileInfo_c *tileInfo, short bitDepth, short wordShift, int dwtlevel,
+                  int decodeLevel, OutputFormat outFormat, StreamId asyncId);
+void BMI_tile_ICT(TileInfo_c *tileInfo, short bitDepth, short wordShift, int dwtlevel,
+                  int decodeLevel, OutputFormat outFormat, StreamId asyncId);
+/* gpulib.h:49 / gpulib.c:292-341: N-component pack without colour
+   transform (CPU MergeComponents semantics) */
+void BMI_tile_Merge(TileInfo_c *tileInfo, int componentNum, int dwtLevel, int decodeLevel,
+                    short bitDepth, short wordShift, int compBufPitchBytes,
+                    OutputFormat outFormat, EncodeMathod method, StreamId asyncId);
+/* gpulib.h:50 / gpulib.c:343-350: grey preview of one component into the
+   32-bit result buffer (0x00RRGGBB words) */
+void BMI_tile_One_Component(TileInfo_c *tile, int component_id, EncodeMathod method, int hasMCT,
+                            int bitDepth, int wordShift, int dwtLevel, int decodeLevel,
+                            StreamId asyncId);
+/* gpulib.h:53-54 / gpulib.c:364-369 -> bmi_dc_shift_kernel: (int)v (9/7)
+   + 2^(B-1), clamp to [0, 2^B-1], int32 out */
+void BMI_Comp_DC_SHIFT(CudaBuf *compBuf, CudaBuf *compBufTemp, EncodeMathod methos,
+                       int src_x_off, int src_y_off, int dst_x_off, int dst_y_off, int width,
+                       int height, int bitDepth, int pixelWordLength, StreamId asyncId);
+
+/* ------------------------------------------------------------------------
+ * Part 2: additive API (no counterpart in the reference)
+ * --------------------------------------------------------------------- */
+
+int BMI_last_error(void);
+const char *BMI_error_string(int code);
+
+/* 5/3 synthesis flavour.  BMI_R53_REFERENCE (default) reproduces the
+   reference CPU decoder bit for bit, including its bottom-edge quirk in
+   the vertical pass (bmi_cpu_dwt_decoder.cpp:3311-3389); where that
+   decoder is undefined (odd parity, length-1 lines) the T.800 result is
+   produced.  BMI_R53_TEXTBOOK is ITU-T T.800 F.3 throughout. */
+#define BMI_R53_REFERENCE 0
+#define BMI_R53_TEXTBOOK 1
+void BMI_set_r53_mode(int mode);
+int BMI_get_r53_mode(void);
+
+/* 9/7 coefficient encodings */
+#define BMI_COEF_SIGNMAG 0 /* T1 sign-magnitude words */
+#define BMI_COEF_FLOAT 1   /* float32 coefficients */
+
+/* One tile (all its components) for the fused / batched decode. */
+typedef struct BmiTileDesc {
+    const void *coef[MAX_COMPONENT]; /* device Mallat planes (32-bit words) */
+    int coef_pitch;                  /* elements */
+    int width, height;               /* tile size at full resolution */
+    int x0, y0;                      /* tile offset in the image (parities) */
+    int levels;                      /* decomposition levels, 0..MAX_DWT_LEVEL */
+    int top_level;                   /* -1 = all; else synthesise this many levels */
+    int ncomp;                       /* 1..4 */
+    int method;                      /* EncodeMathod */
+    int coef_kind;                   /* 9/7: BMI_COEF_SIGNMAG or BMI_COEF_FLOAT */
+    int bit_depth;                   /* 1..16 */
+    int out_format;                  /* OutputFormat; NONE -> float/int planes */
+    int check_out;                   /* -1 = all components, else one as grey */
+    int mct;                         /* 1 = RCT/ICT of components 0..2 */
+    int r53_mode;                    /* BMI_R53_REFERENCE / BMI_R53_TEXTBOOK */
+    float steps[MAX_COMPONENT][3 * MAX_DWT_LEVEL + 1]; /* 9/7 mfAbsStepNorm */
+    void *out;                       /* device packed pixels */
+    int out_pitch_px;                /* pixels */
+    int out_x, out_y;                /* tile position in `out`, pixels */
+    void *planes_out[MAX_COMPONENT]; /* NONE: per-component result planes */
+    int planes_pitch;                /* elements */
+} BmiTileDesc;
+
+/* Launch shape of the fused kernel (0 = library default / autotuned). */
+typedef struct BmiTuning {
+    int seg_rows;        /* output rows per work item (even) */
+    int waves_per_group; /* 1..16 wave64s per workgroup */
+    int groups_per_cu;   /* persistent workgroups per CU */
+    int persistent;      /* 1 = one persistent launch with a device work
+                            queue; 0 = one launch per level step */
+} BmiTuning;
+
+typedef struct BmiBatchPlan BmiBatchPlan;
+
+/* Build a plan for decoding `ntiles` tiles (of any size, level count,
+   filter and component count) in one persistent launch.  Validates the
+   descriptors, builds the device work queue and allocates the scratch it
+   needs.  tuning may be NULL. */
+int BMI_batch_plan_create(BmiBatchPlan **plan, const BmiTileDesc *tiles, int ntiles,
+                          const BmiTuning *tuning);
+/* Enqueue the decode on `stream` (asynchronous). */
+int BMI_batch_plan_execute(BmiBatchPlan *plan, StreamId stream);
+/* Capture execute() in a hipGraph on first use, replay it afterwards. */
+int BMI_batch_plan_execute_graph(BmiBatchPlan *plan, StreamId stream);
+/* Replace device pointers (coefficients / outputs) of tile i keeping its
+   geometry, so one plan (and its graph) serves a stream of same-shaped
+   frames. */
+int BMI_batch_plan_rebind(BmiBatchPlan *plan, int i, const BmiTileDesc *tile);
+size_t BMI_batch_plan_scratch_bytes(const BmiBatchPlan *plan);
+int BMI_batch_plan_num_items(const BmiBatchPlan *plan);
+int BMI_batch_plan_destroy(BmiBatchPlan *plan);
+
+/* Plain path: the reference's own sequence of drop-in calls for one tile
+   (BMI_comp_idwt per component, then BMI_tile_ICT / BMI_tile_MCT /
+   BMI_tile_Merge), on device buffers described by `tile`.  The
+   coefficient planes are overwritten (in-place IDWT).  `temp` must hold
+   width*height*4 bytes (transposed scratch). */
+int BMI_plain_decode(const BmiTileDesc *tile, void *temp, size_t temp_bytes, StreamId stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

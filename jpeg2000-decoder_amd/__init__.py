@@ -1,0 +1,269 @@
+"""Python mirror of the gfx950 back end's C-ABI (include/bmi_gpulib.h).
+
+The product is the C-ABI shared library ``libbmi_j2k.so`` built from
+``csrc/``; this module only binds it with ctypes so tests, the benchmark and
+Python callers can drive it on device buffers they own (torch tensors are
+used for device memory and streams).  Loading fails loudly when the library
+is missing: there is no CPU fallback anywhere in the product path.
+
+Constants and the meaning of every argument follow the reference's own
+interface (decoder/cudainterface/include/gpulib.h:24-57,
+decoder/common/include/codec_datatype.h:16-51).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+
+PKG_DIR = pathlib.Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libbmi_j2k.so"
+
+# EncodeMathod (codec_datatype.h:16-21)
+W5X3 = 0
+W9X7 = 1
+
+# OutputFormat (codec_datatype.h:38-51)
+FMT_32BITS_ARGB = 0
+FMT_48BITS_RGB = 1
+FMT_24BITS_RGB = 2
+FMT_32BITS_4COMPONENTS = 3
+FMT_64BITS_4COMPONENTS = 4
+FMT_48BITS_3COMPONENTS = 5
+FMT_24BITS_3COMPONENTS = 6
+FMT_NONE = 7
+FMT_64BITS_ARGB = 8
+FMT_36BITS_RGB = 9
+FMT_48BITS_ARGB = 10
+
+PIXEL_BYTES = {0: 4, 1: 6, 2: 3, 3: 4, 4: 8, 5: 6, 6: 3, 8: 8}
+RGB_FORMATS = (0, 1, 2, 8)
+MERGE_FORMATS = (3, 4, 5, 6)
+
+COEF_SIGNMAG = 0
+COEF_FLOAT = 1
+R53_REFERENCE = 0
+R53_TEXTBOOK = 1
+
+BMI_OK = 0
+BMI_E_INVALID = -1
+BMI_E_NODEVICE = -2
+BMI_E_HIP = -3
+BMI_E_NOMEM = -4
+BMI_E_UNSUPPORTED = -5
+
+MAX_COMPONENT = 4
+MAX_DWT_LEVEL = 6
+
+
+class BmiTileDesc(ctypes.Structure):
+    """struct BmiTileDesc (include/bmi_gpulib.h)."""
+
+    _fields_ = [
+        ("coef", ctypes.c_void_p * MAX_COMPONENT),
+        ("coef_pitch", ctypes.c_int),
+        ("width", ctypes.c_int),
+        ("height", ctypes.c_int),
+        ("x0", ctypes.c_int),
+        ("y0", ctypes.c_int),
+        ("levels", ctypes.c_int),
+        ("top_level", ctypes.c_int),
+        ("ncomp", ctypes.c_int),
+        ("method", ctypes.c_int),
+        ("coef_kind", ctypes.c_int),
+        ("bit_depth", ctypes.c_int),
+        ("out_format", ctypes.c_int),
+        ("check_out", ctypes.c_int),
+        ("mct", ctypes.c_int),
+        ("r53_mode", ctypes.c_int),
+        ("steps", (ctypes.c_float * (3 * MAX_DWT_LEVEL + 1)) * MAX_COMPONENT),
+        ("out", ctypes.c_void_p),
+        ("out_pitch_px", ctypes.c_int),
+        ("out_x", ctypes.c_int),
+        ("out_y", ctypes.c_int),
+        ("planes_out", ctypes.c_void_p * MAX_COMPONENT),
+        ("planes_pitch", ctypes.c_int),
+    ]
+
+
+class BmiTuning(ctypes.Structure):
+    _fields_ = [
+        ("seg_rows", ctypes.c_int),
+        ("waves_per_group", ctypes.c_int),
+        ("groups_per_cu", ctypes.c_int),
+        ("persistent", ctypes.c_int),
+    ]
+
+
+class GpuInfo(ctypes.Structure):
+    _fields_ = [
+        ("active", ctypes.c_int),
+        ("device", ctypes.c_int),
+        ("name", ctypes.c_char * 256),
+        ("major", ctypes.c_int),
+        ("minor", ctypes.c_int),
+        ("memory", ctypes.c_uint),
+        ("multiProcessorCount", ctypes.c_int),
+        ("clockRate", ctypes.c_int),
+    ]
+
+
+_LIB = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libbmi_j2k.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `make -C {PKG_DIR}` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    L = ctypes.CDLL(str(LIB_PATH))
+    vp, i, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+    L.BMI_last_error.restype = i
+    L.BMI_error_string.restype = ctypes.c_char_p
+    L.BMI_error_string.argtypes = [i]
+    L.BMI_set_r53_mode.argtypes = [i]
+    L.BMI_get_r53_mode.restype = i
+    L.GpuInit.argtypes = [ctypes.POINTER(GpuInfo)]
+    L.GpuInit.restype = i
+    L.GpuRemove.argtypes = [ctypes.POINTER(GpuInfo)]
+    L.GpuRemove.restype = i
+    L.BMI_batch_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BmiTileDesc), i,
+                                        ctypes.POINTER(BmiTuning)]
+    L.BMI_batch_plan_create.restype = i
+    L.BMI_batch_plan_execute.argtypes = [vp, vp]
+    L.BMI_batch_plan_execute.restype = i
+    L.BMI_batch_plan_execute_graph.argtypes = [vp, vp]
+    L.BMI_batch_plan_execute_graph.restype = i
+    L.BMI_batch_plan_rebind.argtypes = [vp, i, ctypes.POINTER(BmiTileDesc)]
+    L.BMI_batch_plan_rebind.restype = i
+    L.BMI_batch_plan_scratch_bytes.argtypes = [vp]
+    L.BMI_batch_plan_scratch_bytes.restype = ctypes.c_size_t
+    L.BMI_batch_plan_num_items.argtypes = [vp]
+    L.BMI_batch_plan_num_items.restype = i
+    L.BMI_batch_plan_destroy.argtypes = [vp]
+    L.BMI_batch_plan_destroy.restype = i
+    L.BMI_batch_plan_error_flag.argtypes = [vp]
+    L.BMI_batch_plan_error_flag.restype = i
+    L.BMI_plain_decode.argtypes = [ctypes.POINTER(BmiTileDesc), vp, ctypes.c_size_t, vp]
+    L.BMI_plain_decode.restype = i
+    _ = f
+    _LIB = L
+    return L
+
+
+class BmiError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str) -> None:
+    if rc != BMI_OK:
+        msg = lib().BMI_error_string(rc).decode()
+        raise BmiError(f"{what}: {msg} ({rc})")
+
+
+def tile_desc(*, coef_ptrs, coef_pitch, width, height, x0=0, y0=0, levels, method,
+              ncomp, coef_kind=COEF_FLOAT, bit_depth=8, out_format=FMT_NONE, check_out=-1,
+              mct=None, r53_mode=R53_REFERENCE, steps=None, out_ptr=0, out_pitch_px=0,
+              out_x=0, out_y=0, planes_ptrs=None, planes_pitch=0, top_level=-1) -> BmiTileDesc:
+    """Fill a BmiTileDesc; steps is [ncomp][3L+1] (defaults to 1.0)."""
+    d = BmiTileDesc()
+    for c, p in enumerate(coef_ptrs):
+        d.coef[c] = p
+    d.coef_pitch = coef_pitch
+    d.width, d.height, d.x0, d.y0 = width, height, x0, y0
+    d.levels, d.top_level, d.ncomp, d.method = levels, top_level, ncomp, method
+    d.coef_kind, d.bit_depth, d.out_format, d.check_out = coef_kind, bit_depth, out_format, check_out
+    d.mct = int(out_format in RGB_FORMATS) if mct is None else int(mct)
+    d.r53_mode = r53_mode
+    for c in range(MAX_COMPONENT):
+        for b in range(3 * MAX_DWT_LEVEL + 1):
+            d.steps[c][b] = 1.0
+    if steps is not None:
+        for c, row in enumerate(steps):
+            for b, v in enumerate(row):
+                d.steps[c][b] = float(v)
+    d.out, d.out_pitch_px, d.out_x, d.out_y = out_ptr or None, out_pitch_px, out_x, out_y
+    if planes_ptrs is not None:
+        for c, p in enumerate(planes_ptrs):
+            d.planes_out[c] = p
+    d.planes_pitch = planes_pitch
+    return d
+
+
+class BatchPlan:
+    """BMI_batch_plan_* wrapper: one persistent launch for many tiles."""
+
+    def __init__(self, descs, tuning: BmiTuning | None = None):
+        L = lib()
+        arr = (BmiTileDesc * len(descs))(*descs)
+        self._descs = arr
+        self.handle = ctypes.c_void_p()
+        rc = L.BMI_batch_plan_create(ctypes.byref(self.handle), arr, len(descs),
+                                     ctypes.byref(tuning) if tuning is not None else None)
+        check(rc, "BMI_batch_plan_create")
+
+    def execute(self, stream: int = 0) -> None:
+        check(lib().BMI_batch_plan_execute(self.handle, stream or None), "BMI_batch_plan_execute")
+
+    def execute_graph(self, stream: int = 0) -> None:
+        check(lib().BMI_batch_plan_execute_graph(self.handle, stream or None),
+              "BMI_batch_plan_execute_graph")
+
+    def rebind(self, i: int, desc: BmiTileDesc) -> None:
+        check(lib().BMI_batch_plan_rebind(self.handle, i, ctypes.byref(desc)), "rebind")
+
+    @property
+    def num_items(self) -> int:
+        return lib().BMI_batch_plan_num_items(self.handle)
+
+    @property
+    def scratch_bytes(self) -> int:
+        return lib().BMI_batch_plan_scratch_bytes(self.handle)
+
+    def error_flag(self) -> int:
+        return lib().BMI_batch_plan_error_flag(self.handle)
+
+    def close(self) -> None:
+        if self.handle:
+            lib().BMI_batch_plan_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plain_decode(desc: BmiTileDesc, temp_ptr: int, temp_bytes: int, stream: int = 0) -> None:
+    """The reference's drop-in call sequence for one tile (in place)."""
+    check(lib().BMI_plain_decode(ctypes.byref(desc), temp_ptr, temp_bytes, stream or None),
+          "BMI_plain_decode")
+
+
+def level_span(t0: int, n: int, levels: int, lev: int):
+    s = levels - 1 - lev
+    return -(-t0 // (1 << s)), -(-(t0 + n) // (1 << s))
+
+
+def tile_geometry(w: int, h: int, x0: int, y0: int, levels: int):
+    """Per synthesis level (0 = coarsest): dict of spans and band sizes."""
+    out = []
+    for lev in range(levels):
+        a0, a1 = level_span(x0, w, levels, lev)
+        b0, b1 = level_span(y0, h, levels, lev)
+        lw = (a1 + 1) // 2 - (a0 + 1) // 2
+        lh = (b1 + 1) // 2 - (b0 + 1) // 2
+        out.append(dict(a0=a0, a1=a1, b0=b0, b1=b1, nw=a1 - a0, nh=b1 - b0, lw=lw, lh=lh))
+    return out
+
+
+def output_size(w: int, h: int, x0: int, y0: int, levels: int, top_level: int = -1):
+    if top_level < 0 or top_level >= levels:
+        return w, h
+    g = tile_geometry(w, h, x0, y0, levels)[top_level - 1]
+    return g["nw"], g["nh"]

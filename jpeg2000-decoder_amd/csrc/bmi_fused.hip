@@ -1,0 +1,869 @@
+// bmi_fused.hip -- fused multi-level IDWT + colour transform + pack for
+// gfx950, batched over any number of tiles in one persistent launch.
+//
+// Replaces the reference's per-level row/column launches
+// (decoder/cudainterface/src/gpufunc.cu:419-546, 953-1089) and its separate
+// colour kernels (gpufunc.cu:1093-1401) with one engine:
+//
+//  * A work item is one wave64 sweeping a vertical strip of one synthesis
+//    level: lane j owns the column pair (2j, 2j+1) of a 128-column window
+//    (60 output pairs for 9/7, 62 for 5/3; the rest is symmetric-extension
+//    or neighbour halo).  Each input row pair (low row from LL|HL, high row
+//    from LH|HH) is loaded coalesced, lifted horizontally with wave
+//    shuffles, and pushed into a per-lane register pipeline that does the
+//    vertical lifting, so a level makes one pass over HBM and uses no LDS.
+//  * The final level of a tile runs all components in lockstep and applies
+//    the inverse RCT/ICT or the merge level shift before packing pixels, so
+//    the last resolution never goes to HBM as float planes.
+//  * Items of all tiles and levels sit in one device work queue; waves take
+//    items in order and wait (bounded spin, acquire) only on the per-job
+//    completion counters of the level below in the same tile.
+//
+// Arithmetic follows the reference CPU back end: gains folded into the row
+// pass as in bmi_cpu_dwt_decoder.cpp:1433-1473, lifting updates
+// x - c*(l + r) rounded per operation, ICT in float
+// (bmi_cpu_dwt_decoder.cpp:3570-3841), RCT with floor (:3936-4130), merge
+// with truncation before the level shift (:4282-4596).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "bmi_device.h"
+#include "bmi_fused.h"
+#include "bmi_internal.h"
+
+namespace bmi {
+
+template <int FILT>
+struct SampleT;
+template <>
+struct SampleT<97> {
+    using T = float;
+};
+template <>
+struct SampleT<53> {
+    using T = int;
+};
+
+template <int FILT>
+__device__ inline typename SampleT<FILT>::T load_sample(const void *base, size_t idx, int kind,
+                                                        float gain)
+{
+    if constexpr (FILT == 97) {
+        uint32_t w = ((const uint32_t *)base)[idx];
+        float v = (kind == SRC_SIGNMAG) ? (float)signmag_to_int(w) : __uint_as_float(w);
+        return __fmul_rn(v, gain);
+    } else {
+        return ((const int *)base)[idx];
+    }
+}
+
+// one horizontal synthesis across the wave: lane holds (low, high) of its
+// column pair; T.800 lifting with the reference's operation order
+template <int FILT, typename T>
+__device__ inline void hlift(T &lo, T &hi)
+{
+    if constexpr (FILT == 97) {
+        float hl = wave_from_left(hi);
+        lo = lift97(lo, kDelta, hl, hi);
+        float lr = wave_from_right(lo);
+        hi = lift97(hi, kGamma, lo, lr);
+        hl = wave_from_left(hi);
+        lo = lift97(lo, kBeta, hl, hi);
+        lr = wave_from_right(lo);
+        hi = lift97(hi, kAlpha, lo, lr);
+    } else {
+        int hl = wave_from_left(hi);
+        lo = lo - ((hl + hi + 2) >> 2);
+        int lr = wave_from_right(lo);
+        hi = hi + ((lo + lr) >> 1);
+    }
+}
+
+// single-sample line: T.800 F.3.7 (X = Y, or Y/2 for an odd position)
+template <int FILT, typename T>
+__device__ inline T half_sample(T v)
+{
+    if constexpr (FILT == 97)
+        return __fmul_rn(v, 0.5f);
+    else
+        return v >> 1;
+}
+
+// write one output position (all lockstep components of one pixel)
+template <int FILT, int NC>
+__device__ inline void emit_pixel(const LevelJob &J, int row, int col,
+                                  const typename SampleT<FILT>::T *v)
+{
+    using T = typename SampleT<FILT>::T;
+    const int ok = J.out_kind;
+    if (ok == OUT_SCRATCH || ok == OUT_PLANE) {
+        size_t idx = (size_t)(row - J.b0) * J.dst_pitch + (col - J.a0);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            if (c < J.nc) ((T *)J.dst[c])[idx] = v[c];
+        return;
+    }
+    const int bd = J.bit_depth;
+    const int od = fmt_depth(J.fmt);
+    int R = 0, G = 0, B = 0, A = 0;
+    if (ok == OUT_RGB) {
+        if constexpr (NC >= 3) {
+            if constexpr (FILT == 97)
+                ict_pixel(v[0], v[1], v[2], bd, od, J.check_out, R, G, B);
+            else
+                rct_pixel(v[0], v[1], v[2], bd, od, J.check_out, R, G, B);
+        }
+    } else {  // OUT_MERGE
+        int t[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if (c < J.nc) {
+                int s;
+                if constexpr (FILT == 97) s = (int)v[c];
+                else s = v[c];
+                t[c] = merge_value(s, bd, od);
+            }
+        }
+        merge_slots(t, J.fmt, J.check_out == 0, R, G, B, A);
+    }
+    uint8_t *p = (uint8_t *)J.out +
+                 ((size_t)(J.out_y + row - J.b0) * J.out_pitch_px + (J.out_x + col - J.a0)) *
+                     fmt_bytes(J.fmt);
+    put_pixel(p, J.fmt, R, G, B, A);
+}
+
+// One work item: strip `strip`, row segment `seg` of level job J, run by
+// one wave64 (lane = 0..63).
+template <int FILT, int NC>
+__device__ void process_item(const LevelJob &J, int strip, int seg, int lane)
+{
+    using T = typename SampleT<FILT>::T;
+    constexpr int HP = (FILT == 97) ? 2 : 1;  // halo pairs on each side
+    constexpr int OWP = 64 - 2 * HP;          // output pairs per strip
+
+    const int a0 = J.a0, a1 = J.a1, b0 = J.b0, b1 = J.b1;
+    const bool single_col = (a1 - a0) == 1;
+    const bool single_row = (b1 - b0) == 1;
+
+    // absolute columns of this lane (even = low sample, odd = high sample)
+    const int wpair0 = ((a0 & ~1) >> 1) + strip * OWP - HP;
+    const int ce = 2 * (wpair0 + lane), co = ce + 1;
+    const bool lane_out = (lane >= HP) && (lane < 64 - HP);
+    int lo_idx = 0, hi_idx = 0;
+    if (!single_col) {
+        const int rce = reflect(ce, a0, a1), rco = reflect(co, a0, a1);
+        lo_idx = (rce >> 1) - J.cl0;
+        hi_idx = (rco >> 1) - J.ch0;
+    }
+    const bool col_is_low = single_col && !(a0 & 1);
+
+    // row pairs of this segment
+    const int K0 = (b0 & ~1) >> 1;
+    const int Klast = (b1 - 1) >> 1;
+    const int kout0 = K0 + seg * J.seg_pairs;
+    const int kout1 = min(Klast, kout0 + J.seg_pairs - 1);
+
+    // load and horizontally synthesise the two rows of pair k for component
+    // c: (tl, th) = low row at columns (ce, co), (bl, bh) = high row
+    auto load_rows = [&](int k, int c, T &tl, T &th, T &bl, T &bh) {
+        int rre, rro;
+        if (!single_row) {
+            rre = reflect(2 * k, b0, b1);
+            rro = reflect(2 * k + 1, b0, b1);
+        } else {
+            rre = rro = b0;
+        }
+        const int r = (rre >> 1) - J.rl0;          // row in the low half
+        const int q = (rro >> 1) - J.rh0 + J.lh;   // plane row of the high half
+        const float *g = J.gain[c];
+        const bool has_top = !single_row || !(b0 & 1);
+        const bool has_bot = !single_row || (b0 & 1);
+        tl = th = bl = bh = (T)0;
+        if (!single_col) {
+            if (has_top) {
+                tl = load_sample<FILT>(J.ll[c], (size_t)r * J.ll_pitch + lo_idx, J.ll_kind, g[0]);
+                th = load_sample<FILT>(J.det[c], (size_t)r * J.det_pitch + J.lw + hi_idx,
+                                       J.det_kind, g[1]);
+            }
+            if (has_bot) {
+                bl = load_sample<FILT>(J.det[c], (size_t)q * J.det_pitch + lo_idx, J.det_kind,
+                                       g[2]);
+                bh = load_sample<FILT>(J.det[c], (size_t)q * J.det_pitch + J.lw + hi_idx,
+                                       J.det_kind, g[3]);
+            }
+            hlift<FILT>(tl, th);
+            hlift<FILT>(bl, bh);
+        } else {
+            if (col_is_low) {
+                if (has_top)
+                    tl = load_sample<FILT>(J.ll[c], (size_t)r * J.ll_pitch, J.ll_kind, g[0]);
+                if (has_bot)
+                    bl = load_sample<FILT>(J.det[c], (size_t)q * J.det_pitch, J.det_kind, g[2]);
+            } else {
+                if (has_top)
+                    tl = half_sample<FILT>(load_sample<FILT>(
+                        J.det[c], (size_t)r * J.det_pitch + J.lw, J.det_kind, g[1]));
+                if (has_bot)
+                    bl = half_sample<FILT>(load_sample<FILT>(
+                        J.det[c], (size_t)q * J.det_pitch + J.lw, J.det_kind, g[3]));
+            }
+            th = tl;
+            bh = bl;
+        }
+    };
+
+    // ve = values at (row, ce), vo = values at (row, co), NC components
+    auto emit_row = [&](int row, const T *ve, const T *vo) {
+        if (!lane_out || row < b0 || row >= b1) return;
+        if (ce >= a0 && ce < a1) emit_pixel<FILT, NC>(J, row, ce, ve);
+        if (co >= a0 && co < a1) emit_pixel<FILT, NC>(J, row, co, vo);
+    };
+
+    if (single_row) {
+        T ve[NC], vo[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            T tl = 0, th = 0, bl = 0, bh = 0;
+            if (c < J.nc) load_rows(b0 >> 1, c, tl, th, bl, bh);
+            if (!(b0 & 1)) {
+                ve[c] = tl;
+                vo[c] = th;
+            } else {
+                ve[c] = half_sample<FILT>(bl);
+                vo[c] = half_sample<FILT>(bh);
+            }
+        }
+        emit_row(b0, ve, vo);
+        return;
+    }
+
+    if constexpr (FILT == 97) {
+        // per component and column (0 = ce, 1 = co): H_{k-1}, e3_{k-1},
+        // o4_{k-2}, e5_{k-2}; pair k in, pair k-2 out
+        float Hp[NC][2], e3p[NC][2], o4pp[NC][2], e5pp[NC][2];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) Hp[c][e] = e3p[c][e] = o4pp[c][e] = e5pp[c][e] = 0.0f;
+        const int ks = kout0 - 2, ke = kout1 + 2;
+        for (int k = ks; k <= ke; ++k) {
+            float evn[2][NC], odd[2][NC];  // [column][component]
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                float tl = 0.f, th = 0.f, bl = 0.f, bh = 0.f;
+                if (c < J.nc) load_rows(k, c, tl, th, bl, bh);
+                const float Lk[2] = {tl, th}, Hk[2] = {bl, bh};
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const float e3 = lift97(Lk[e], kDelta, Hp[c][e], Hk[e]);
+                    const float o4 = lift97(Hp[c][e], kGamma, e3p[c][e], e3);
+                    const float e5 = lift97(e3p[c][e], kBeta, o4pp[c][e], o4);
+                    const float o6 = lift97(o4pp[c][e], kAlpha, e5pp[c][e], e5);
+                    evn[e][c] = e5pp[c][e];
+                    odd[e][c] = o6;
+                    Hp[c][e] = Hk[e];
+                    e3p[c][e] = e3;
+                    o4pp[c][e] = o4;
+                    e5pp[c][e] = e5;
+                }
+            }
+            const int ko = k - 2;
+            if (ko >= kout0) {
+                emit_row(2 * ko, evn[0], evn[1]);
+                emit_row(2 * ko + 1, odd[0], odd[1]);
+            }
+        }
+    } else {
+        // per component and column: H_{k-1}, e1_{k-1}, plus L_{k-1} and
+        // H_{k-2} for the reference bottom-edge quirk; pair k in, k-1 out
+        int Hp[NC][2], e1p[NC][2], Lp[NC][2], Hpp[NC][2];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) Hp[c][e] = e1p[c][e] = Lp[c][e] = Hpp[c][e] = 0;
+        const int nh = b1 - b0;
+        // iteration at which the reference's VDWT_R53_LINES tail applies:
+        // local pair nh/2 (bmi_cpu_dwt_decoder.cpp:3359-3378)
+        const int kq = J.quirk ? (K0 + (nh >> 1)) : -0x7fffffff;
+        const bool nh_even = !(nh & 1);
+        const int ks = kout0 - (J.quirk ? 2 : 1), ke = kout1 + 1;
+        for (int k = ks; k <= ke; ++k) {
+            int evn[2][NC], odd[2][NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                int tl = 0, th = 0, bl = 0, bh = 0;
+                if (c < J.nc) load_rows(k, c, tl, th, bl, bh);
+                const int Lk[2] = {tl, th}, Hk[2] = {bl, bh};
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    int e1 = Lk[e] - ((Hp[c][e] + Hk[e] + 2) >> 2);
+                    int o2 = Hp[c][e] + ((e1p[c][e] + e1) >> 1);
+                    if (k == kq) {
+                        if (nh_even) {
+                            // last odd row uses the stale high row H_{m-2}
+                            o2 = Hk[e] + e1p[c][e];
+                        } else {
+                            // last two rows use L_{m-1} and H_{m-2}
+                            e1 = Lp[c][e] - ((Hpp[c][e] + Hpp[c][e] + 2) >> 2);
+                            o2 = Hpp[c][e] + ((e1p[c][e] + e1) >> 1);
+                        }
+                    }
+                    evn[e][c] = e1p[c][e];
+                    odd[e][c] = o2;
+                    Hpp[c][e] = Hp[c][e];
+                    Hp[c][e] = Hk[e];
+                    Lp[c][e] = Lk[e];
+                    e1p[c][e] = e1;
+                }
+            }
+            const int ko = k - 1;
+            if (ko >= kout0) {
+                emit_row(2 * ko, evn[0], evn[1]);
+                emit_row(2 * ko + 1, odd[0], odd[1]);
+            }
+        }
+    }
+}
+
+template <int FILT>
+__device__ inline void dispatch_nc(const LevelJob &J, int strip, int seg, int lane)
+{
+    if (J.nc == 1)
+        process_item<FILT, 1>(J, strip, seg, lane);
+    else if (J.nc <= 3)
+        process_item<FILT, 3>(J, strip, seg, lane);
+    else
+        process_item<FILT, 4>(J, strip, seg, lane);
+}
+
+__device__ inline void run_item(const LevelJob *jobs, const WorkItem &it, int lane)
+{
+    const LevelJob &J = jobs[it.job];
+    const int strip = it.local % J.nstrips;
+    const int seg = it.local / J.nstrips;
+    if (J.filt == 97)
+        dispatch_nc<97>(J, strip, seg, lane);
+    else
+        dispatch_nc<53>(J, strip, seg, lane);
+}
+
+// spin limit: a dependency that never completes (a bug) ends the wait and
+// raises the error flag instead of hanging the device
+constexpr int kSpinLimit = 1 << 22;
+
+// Counter block layout: [0, njobs) per-job completion counts, then the
+// queue head, the count of exited waves and the sticky error flag.  The
+// last wave to leave zeroes the counts, head and exit count, so the next
+// launch (e.g. a graph replay) starts from a clean queue without a
+// separate reset kernel.
+__global__ void __launch_bounds__(1024) fused_persistent_kernel(const LevelJob *jobs,
+                                                                const WorkItem *items, int n_items,
+                                                                int *counters, int njobs,
+                                                                int total_waves)
+{
+    int *head = counters + njobs;
+    int *exited = counters + njobs + 1;
+    int *error_flag = counters + njobs + 2;
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        int i = 0;
+        if (lane == 0) i = atomicAdd(head, 1);
+        i = __builtin_amdgcn_readfirstlane(__shfl(i, 0, 64));
+        if (i >= n_items) break;
+        const WorkItem it = items[i];
+        const LevelJob &J = jobs[it.job];
+        if (lane == 0) {
+            for (int d = 0; d < J.ndeps; ++d) {
+                const int dj = J.deps[d];
+                const int need = jobs[dj].n_items;
+                int spins = 0;
+                while (__hip_atomic_load(&counters[dj], __ATOMIC_ACQUIRE,
+                                         __HIP_MEMORY_SCOPE_AGENT) < need) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > kSpinLimit) {
+                        atomicOr(error_flag, 1);
+                        break;
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        run_item(jobs, it, lane);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (lane == 0)
+            __hip_atomic_fetch_add(&counters[it.job], 1, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+    }
+    int last = 0;
+    if (lane == 0)
+        last = __hip_atomic_fetch_add(exited, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+               total_waves - 1;
+    last = __shfl(last, 0, 64);
+    if (last) {
+        for (int j = lane; j < njobs; j += 64)
+            __hip_atomic_store(&counters[j], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) {
+            __hip_atomic_store(head, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(exited, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    }
+}
+
+// level-step mode: one launch per dependency step, no waiting
+__global__ void __launch_bounds__(1024) fused_step_kernel(const LevelJob *jobs,
+                                                          const WorkItem *items, int first,
+                                                          int count)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (i >= count) return;
+    run_item(jobs, items[first + i], lane);
+}
+
+}  // namespace bmi
+
+// ---------------------------------------------------------------------------
+// host side: plans
+// ---------------------------------------------------------------------------
+
+using namespace bmi;
+
+struct BmiBatchPlan {
+    std::vector<BmiTileDesc> tiles;
+    std::vector<LevelJob> jobs;
+    std::vector<WorkItem> items;
+    std::vector<int> step_first, step_count;  // level-step mode
+    std::vector<int> job_tile;                // tile of each job
+    BmiTuning tuning;
+    LevelJob *d_jobs = nullptr;
+    WorkItem *d_items = nullptr;
+    int *d_counters = nullptr;  // [njobs] completion counters, [njobs] = head,
+                                // [njobs+1] = error flag
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    std::vector<size_t> tile_scratch_off;  // per tile: base offset
+    std::vector<int> tile_scratch_pitch;
+    std::vector<size_t> tile_scratch_plane;  // bytes per plane
+    int device = 0;
+    int num_cus = 256;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+};
+
+namespace {
+
+int out_kind_for(const BmiTileDesc &t)
+{
+    switch (t.out_format) {
+    case IMAGE_FOTMAT_NONE:
+        return OUT_PLANE;
+    case IMAGE_FOTMAT_32BITS_ARGB:
+    case IMAGE_FOTMAT_48BITS_RGB:
+    case IMAGE_FOTMAT_24BITS_RGB:
+    case IMAGE_FOTMAT_64BITS_ARGB:
+        return OUT_RGB;
+    case IMAGE_FOTMAT_32BITS_4COMPONENTS:
+    case IMAGE_FOTMAT_64BITS_4COMPONENTS:
+    case IMAGE_FOTMAT_48BITS_3COMPONENTS:
+    case IMAGE_FOTMAT_24BITS_3COMPONENTS:
+        return OUT_MERGE;
+    default:
+        return -1;
+    }
+}
+
+int validate_tile(const BmiTileDesc &t)
+{
+    if (t.width < 1 || t.height < 1 || t.width > 32767 || t.height > 32767) return BMI_E_INVALID;
+    if (t.x0 < 0 || t.y0 < 0) return BMI_E_INVALID;
+    if (t.levels < 1 || t.levels > MAX_DWT_LEVEL) return BMI_E_INVALID;
+    if (t.top_level == 0) return BMI_E_UNSUPPORTED;
+    if (t.top_level < -1 || t.top_level > t.levels) return BMI_E_INVALID;
+    if (t.ncomp < 1 || t.ncomp > MAX_COMPONENT) return BMI_E_INVALID;
+    if (t.method != Ckernels_W5X3 && t.method != Ckernels_W9X7) return BMI_E_INVALID;
+    if (t.method == Ckernels_W9X7 && t.coef_kind != BMI_COEF_SIGNMAG &&
+        t.coef_kind != BMI_COEF_FLOAT)
+        return BMI_E_INVALID;
+    if (t.bit_depth < 1 || t.bit_depth > 16) return BMI_E_INVALID;
+    if (t.coef_pitch < t.width) return BMI_E_INVALID;
+    const int ok = out_kind_for(t);
+    if (ok < 0) return BMI_E_UNSUPPORTED;
+    if (ok == OUT_RGB && t.ncomp < 3) return BMI_E_UNSUPPORTED;
+    if (ok == OUT_RGB && (t.check_out < -1 || t.check_out > 2)) return BMI_E_INVALID;
+    if (ok == OUT_MERGE && (t.check_out < -1 || t.check_out >= t.ncomp)) return BMI_E_INVALID;
+    if (ok == OUT_PLANE) {
+        for (int c = 0; c < t.ncomp; ++c)
+            if (!t.planes_out[c]) return BMI_E_INVALID;
+    } else if (!t.out || t.out_pitch_px < 1 || t.out_x < 0 || t.out_y < 0) {
+        return BMI_E_INVALID;
+    }
+    if (t.r53_mode != BMI_R53_REFERENCE && t.r53_mode != BMI_R53_TEXTBOOK) return BMI_E_INVALID;
+    return BMI_OK;
+}
+
+void level_span(int t0, int n, int L, int lev, int *a0, int *a1)
+{
+    const int s = L - 1 - lev;
+    *a0 = ceil_shift(t0, s);
+    *a1 = ceil_shift(t0 + n, s);
+}
+
+// components whose planes the final level reads
+void needed_comps(const BmiTileDesc &t, int ok, int *list, int *n)
+{
+    *n = 0;
+    if (ok == OUT_RGB) {
+        for (int c = 0; c < 3; ++c) list[(*n)++] = c;
+    } else if (ok == OUT_MERGE && t.check_out >= 0) {
+        list[(*n)++] = t.check_out;
+    } else {
+        for (int c = 0; c < t.ncomp; ++c) list[(*n)++] = c;
+    }
+}
+
+}  // namespace
+
+static int default_seg_rows(const BmiTuning *t) { return (t && t->seg_rows > 0) ? t->seg_rows : 32; }
+
+extern "C" int BMI_batch_plan_create(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles,
+                                     const BmiTuning *tuning)
+{
+    if (!out_plan || !tiles || ntiles < 1) return bmi_fail(BMI_E_INVALID, "batch_plan_create: bad args");
+    *out_plan = nullptr;
+    for (int i = 0; i < ntiles; ++i) {
+        int rc = validate_tile(tiles[i]);
+        if (rc) return bmi_fail(rc, "batch_plan_create: tile %d rejected", i);
+    }
+    BmiBatchPlan *P = new BmiBatchPlan();
+    P->tiles.assign(tiles, tiles + ntiles);
+    P->tuning.seg_rows = default_seg_rows(tuning);
+    if (P->tuning.seg_rows & 1) P->tuning.seg_rows += 1;
+    P->tuning.waves_per_group = (tuning && tuning->waves_per_group > 0) ? tuning->waves_per_group : 4;
+    P->tuning.groups_per_cu = (tuning && tuning->groups_per_cu > 0) ? tuning->groups_per_cu : 8;
+    P->tuning.persistent = tuning ? tuning->persistent : 1;
+    if (P->tuning.waves_per_group > 16) P->tuning.waves_per_group = 16;
+    if (hipGetDevice(&P->device) != hipSuccess) P->device = 0;
+    {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device) ==
+                hipSuccess &&
+            cus > 0)
+            P->num_cus = cus;
+    }
+
+    // scratch layout: per tile, per needed component, two planes sized for
+    // the largest intermediate level
+    size_t total = 0;
+    P->tile_scratch_off.resize(ntiles);
+    P->tile_scratch_pitch.resize(ntiles);
+    P->tile_scratch_plane.resize(ntiles);
+    for (int ti = 0; ti < ntiles; ++ti) {
+        const BmiTileDesc &t = tiles[ti];
+        const int L = t.levels;
+        const int Ls = (t.top_level < 0) ? L : t.top_level;
+        int maxw = 0, maxh = 0;
+        for (int lev = 0; lev + 1 < Ls; ++lev) {
+            int a0, a1, b0, b1;
+            level_span(t.x0, t.width, L, lev, &a0, &a1);
+            level_span(t.y0, t.height, L, lev, &b0, &b1);
+            maxw = std::max(maxw, a1 - a0);
+            maxh = std::max(maxh, b1 - b0);
+        }
+        const int pitch = (maxw + 63) & ~63;
+        const size_t plane = (size_t)pitch * maxh * 4;
+        P->tile_scratch_off[ti] = total;
+        P->tile_scratch_pitch[ti] = pitch;
+        P->tile_scratch_plane[ti] = plane;
+        total += plane * 2 * (size_t)t.ncomp;
+        total = (total + 255) & ~(size_t)255;
+    }
+    P->scratch_bytes = total;
+    if (total) {
+        if (hipMalloc(&P->scratch, total) != hipSuccess) {
+            delete P;
+            return bmi_fail(BMI_E_NOMEM, "batch_plan_create: scratch of %zu bytes", total);
+        }
+    }
+
+    // jobs, ordered by level step then tile; items follow job order
+    int max_steps = 0;
+    for (int ti = 0; ti < ntiles; ++ti)
+        max_steps = std::max(max_steps, tiles[ti].top_level < 0 ? tiles[ti].levels : tiles[ti].top_level);
+    std::vector<std::vector<int>> job_of(ntiles);  // [tile][lev*4+comp] -> job
+    for (int ti = 0; ti < ntiles; ++ti) job_of[ti].assign(MAX_DWT_LEVEL * 4, -1);
+    const int seg_rows = P->tuning.seg_rows;
+    P->step_first.assign(max_steps, 0);
+    P->step_count.assign(max_steps, 0);
+    for (int step = 0; step < max_steps; ++step) {
+        P->step_first[step] = (int)P->items.size();
+        for (int ti = 0; ti < ntiles; ++ti) {
+            const BmiTileDesc &t = tiles[ti];
+            const int L = t.levels;
+            const int Ls = (t.top_level < 0) ? L : t.top_level;
+            const int lev = step;
+            if (lev >= Ls) continue;
+            const bool final_level = (lev == Ls - 1);
+            const int ok = final_level ? out_kind_for(t) : (int)OUT_SCRATCH;
+            const int filt = (t.method == Ckernels_W9X7) ? 97 : 53;
+            int a0, a1, b0, b1;
+            level_span(t.x0, t.width, L, lev, &a0, &a1);
+            level_span(t.y0, t.height, L, lev, &b0, &b1);
+            LevelJob base;
+            memset(&base, 0, sizeof(base));
+            base.a0 = a0;
+            base.a1 = a1;
+            base.b0 = b0;
+            base.b1 = b1;
+            base.lw = ceil_div2(a1) - ceil_div2(a0);
+            base.lh = ceil_div2(b1) - ceil_div2(b0);
+            base.cl0 = ceil_div2(a0);
+            base.ch0 = floor_div2(a0);
+            base.rl0 = ceil_div2(b0);
+            base.rh0 = floor_div2(b0);
+            const int owp = (filt == 97) ? 60 : 62;
+            const int ncols = a1 - (a0 & ~1);
+            base.nstrips = (ncols + 2 * owp - 1) / (2 * owp);
+            const int nrows = b1 - (b0 & ~1);
+            base.seg_pairs = seg_rows / 2;
+            base.nsegs = (nrows + seg_rows - 1) / seg_rows;
+            base.filt = filt;
+            base.lev = lev;
+            base.out_kind = ok;
+            base.quirk = (filt == 53 && t.r53_mode == BMI_R53_REFERENCE && !(b0 & 1) &&
+                          (b1 - b0) >= 2)
+                             ? 1
+                             : 0;
+            base.det_kind = (filt == 97) ? (t.coef_kind == BMI_COEF_FLOAT ? SRC_FLOAT : SRC_SIGNMAG)
+                                         : SRC_INT;
+            base.ll_kind = (lev == 0) ? base.det_kind : (filt == 97 ? SRC_FLOAT : SRC_INT);
+            base.det_pitch = t.coef_pitch;
+            base.ll_pitch = (lev == 0) ? t.coef_pitch : P->tile_scratch_pitch[ti];
+            base.n_items = base.nstrips * base.nsegs;
+            base.fmt = t.out_format;
+            base.bit_depth = t.bit_depth;
+            base.check_out = t.check_out;
+            base.out = t.out;
+            base.out_pitch_px = t.out_pitch_px;
+            base.out_x = t.out_x;
+            base.out_y = t.out_y;
+
+            auto scratch_plane = [&](int c, int which) -> void * {
+                return (char *)P->scratch + P->tile_scratch_off[ti] +
+                       ((size_t)c * 2 + which) * P->tile_scratch_plane[ti];
+            };
+            auto fill_comp = [&](LevelJob &J, int slot, int c) {
+                J.comp[slot] = c;
+                J.det[slot] = t.coef[c];
+                J.ll[slot] = (lev == 0) ? t.coef[c] : scratch_plane(c, (lev - 1) & 1);
+                const float *s = t.steps[c];
+                if (filt == 97) {
+                    J.gain[slot][0] = lev == 0 ? (s[0] * kLoGain) * kLoGain : kLoGain * kLoGain;
+                    J.gain[slot][1] = (s[3 * lev + 1] * kHiGain) * kLoGain;
+                    J.gain[slot][2] = (s[3 * lev + 2] * kLoGain) * kHiGain;
+                    J.gain[slot][3] = (s[3 * lev + 3] * kHiGain) * kHiGain;
+                } else {
+                    J.gain[slot][0] = J.gain[slot][1] = J.gain[slot][2] = J.gain[slot][3] = 1.0f;
+                }
+            };
+            auto add_deps = [&](LevelJob &J, int c) {
+                if (lev == 0) return;
+                const int dj = job_of[ti][(lev - 1) * 4 + c];
+                if (dj >= 0 && J.ndeps < kMaxDeps) J.deps[J.ndeps++] = dj;
+            };
+            auto push_job = [&](const LevelJob &J, int key_comp) {
+                const int jid = (int)P->jobs.size();
+                P->jobs.push_back(J);
+                P->job_tile.push_back(ti);
+                if (key_comp >= 0) job_of[ti][lev * 4 + key_comp] = jid;
+                for (int li = 0; li < J.n_items; ++li) P->items.push_back(WorkItem{jid, li});
+                return jid;
+            };
+
+            int comps[4], ncomps = 0;
+            needed_comps(t, final_level ? ok : (int)OUT_PLANE, comps, &ncomps);
+            if (!final_level) {
+                // one job per component the final level will need
+                int fcomps[4], nf = 0;
+                needed_comps(t, out_kind_for(t), fcomps, &nf);
+                for (int i = 0; i < nf; ++i) {
+                    const int c = fcomps[i];
+                    LevelJob J = base;
+                    J.nc = 1;
+                    fill_comp(J, 0, c);
+                    J.dst[0] = scratch_plane(c, lev & 1);
+                    J.dst_pitch = P->tile_scratch_pitch[ti];
+                    add_deps(J, c);
+                    push_job(J, c);
+                }
+            } else if (ok == OUT_PLANE) {
+                for (int i = 0; i < ncomps; ++i) {
+                    const int c = comps[i];
+                    LevelJob J = base;
+                    J.nc = 1;
+                    fill_comp(J, 0, c);
+                    J.dst[0] = t.planes_out[c];
+                    J.dst_pitch = t.planes_pitch;
+                    add_deps(J, c);
+                    push_job(J, c);
+                }
+            } else {
+                LevelJob J = base;
+                J.nc = ncomps;
+                for (int i = 0; i < ncomps; ++i) {
+                    fill_comp(J, i, comps[i]);
+                    add_deps(J, comps[i]);
+                }
+                // a merge that checks out one component shows it as grey
+                if (ok == OUT_MERGE && t.check_out >= 0) J.check_out = 0;
+                push_job(J, -1);
+            }
+        }
+        P->step_count[step] = (int)P->items.size() - P->step_first[step];
+    }
+
+    const size_t jb = sizeof(LevelJob) * P->jobs.size();
+    const size_t ib = sizeof(WorkItem) * P->items.size();
+    const size_t cb = sizeof(int) * (P->jobs.size() + 3);
+    if (hipMalloc(&P->d_jobs, jb) != hipSuccess || hipMalloc(&P->d_items, ib) != hipSuccess ||
+        hipMalloc(&P->d_counters, cb) != hipSuccess) {
+        BMI_batch_plan_destroy(P);
+        return bmi_fail(BMI_E_NOMEM, "batch_plan_create: descriptor allocation");
+    }
+    if (hipMemcpy(P->d_jobs, P->jobs.data(), jb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(P->d_items, P->items.data(), ib, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(P->d_counters, 0, cb) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        BMI_batch_plan_destroy(P);
+        return bmi_fail(BMI_E_HIP, "batch_plan_create: descriptor upload");
+    }
+    *out_plan = P;
+    return BMI_OK;
+}
+
+static int plan_enqueue(BmiBatchPlan *P, hipStream_t s)
+{
+    const int nj = (int)P->jobs.size();
+    const int n = (int)P->items.size();
+    if (n == 0) return BMI_OK;
+    const int wpg = P->tuning.waves_per_group;
+    if (P->tuning.persistent) {
+        const int max_groups = P->num_cus * P->tuning.groups_per_cu;
+        const int need_groups = (n + wpg - 1) / wpg;
+        const int groups = std::max(1, std::min(max_groups, need_groups));
+        hipLaunchKernelGGL(fused_persistent_kernel, dim3(groups), dim3(64 * wpg), 0, s, P->d_jobs,
+                           P->d_items, n, P->d_counters, nj, groups * wpg);
+    } else {
+        for (size_t st = 0; st < P->step_first.size(); ++st) {
+            const int cnt = P->step_count[st];
+            if (!cnt) continue;
+            hipLaunchKernelGGL(fused_step_kernel, dim3((cnt + wpg - 1) / wpg), dim3(64 * wpg), 0, s,
+                               P->d_jobs, P->d_items, P->step_first[st], cnt);
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return bmi_fail(BMI_E_HIP, "fused launch: %s", hipGetErrorString(e));
+    return BMI_OK;
+}
+
+extern "C" int BMI_batch_plan_execute(BmiBatchPlan *P, StreamId stream)
+{
+    if (!P) return bmi_fail(BMI_E_INVALID, "batch_plan_execute: null plan");
+    return plan_enqueue(P, (hipStream_t)stream);
+}
+
+extern "C" int BMI_batch_plan_execute_graph(BmiBatchPlan *P, StreamId stream)
+{
+    if (!P) return bmi_fail(BMI_E_INVALID, "batch_plan_execute_graph: null plan");
+    hipStream_t s = (hipStream_t)stream;
+    if (!P->graph_exec) {
+        hipStream_t cap;
+        if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess)
+            return bmi_fail(BMI_E_HIP, "graph: stream");
+        if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipStreamDestroy(cap);
+            return bmi_fail(BMI_E_HIP, "graph: begin capture");
+        }
+        int rc = plan_enqueue(P, cap);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(cap, &g);
+        (void)hipStreamDestroy(cap);
+        if (rc || e != hipSuccess || !g) return bmi_fail(BMI_E_HIP, "graph: capture failed");
+        if (hipGraphInstantiate(&P->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            P->graph_exec = nullptr;
+            return bmi_fail(BMI_E_HIP, "graph: instantiate");
+        }
+        P->graph = g;
+    }
+    if (hipGraphLaunch(P->graph_exec, s) != hipSuccess) return bmi_fail(BMI_E_HIP, "graph: launch");
+    return BMI_OK;
+}
+
+extern "C" int BMI_batch_plan_rebind(BmiBatchPlan *P, int i, const BmiTileDesc *t)
+{
+    if (!P || !t || i < 0 || i >= (int)P->tiles.size())
+        return bmi_fail(BMI_E_INVALID, "batch_plan_rebind: bad args");
+    const BmiTileDesc &o = P->tiles[i];
+    if (t->width != o.width || t->height != o.height || t->levels != o.levels ||
+        t->ncomp != o.ncomp || t->method != o.method || t->out_format != o.out_format ||
+        t->top_level != o.top_level || t->x0 != o.x0 || t->y0 != o.y0)
+        return bmi_fail(BMI_E_INVALID, "batch_plan_rebind: geometry differs");
+    int rc = validate_tile(*t);
+    if (rc) return bmi_fail(rc, "batch_plan_rebind: tile rejected");
+    for (size_t j = 0; j < P->jobs.size(); ++j) {
+        if (P->job_tile[j] != i) continue;
+        LevelJob &J = P->jobs[j];
+        for (int s = 0; s < J.nc; ++s) {
+            const int c = J.comp[s];
+            J.det[s] = t->coef[c];
+            if (J.lev == 0) J.ll[s] = t->coef[c];
+            if (J.out_kind == OUT_PLANE) J.dst[s] = t->planes_out[c];
+        }
+        if (J.out_kind == OUT_RGB || J.out_kind == OUT_MERGE) {
+            J.out = t->out;
+            J.out_pitch_px = t->out_pitch_px;
+            J.out_x = t->out_x;
+            J.out_y = t->out_y;
+        }
+        if (J.out_kind == OUT_PLANE) J.dst_pitch = t->planes_pitch;
+    }
+    P->tiles[i] = *t;
+    if (hipMemcpy(P->d_jobs, P->jobs.data(), sizeof(LevelJob) * P->jobs.size(),
+                  hipMemcpyHostToDevice) != hipSuccess)
+        return bmi_fail(BMI_E_HIP, "batch_plan_rebind: upload");
+    return BMI_OK;
+}
+
+extern "C" size_t BMI_batch_plan_scratch_bytes(const BmiBatchPlan *P)
+{
+    return P ? P->scratch_bytes : 0;
+}
+
+extern "C" int BMI_batch_plan_num_items(const BmiBatchPlan *P) { return P ? (int)P->items.size() : 0; }
+
+extern "C" int BMI_batch_plan_destroy(BmiBatchPlan *P)
+{
+    if (!P) return BMI_OK;
+    if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    if (P->d_jobs) (void)hipFree(P->d_jobs);
+    if (P->d_items) (void)hipFree(P->d_items);
+    if (P->d_counters) (void)hipFree(P->d_counters);
+    if (P->scratch) (void)hipFree(P->scratch);
+    delete P;
+    return BMI_OK;
+}
+
+extern "C" int BMI_batch_plan_error_flag(BmiBatchPlan *P)
+{
+    if (!P || !P->d_counters) return 0;
+    int v = 0;
+    (void)hipMemcpy(&v, P->d_counters + P->jobs.size() + 2, sizeof(int), hipMemcpyDeviceToHost);
+    return v;
+}

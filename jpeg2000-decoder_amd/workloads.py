@@ -1,0 +1,176 @@
+"""Deterministic synthetic workloads (coefficient planes in Mallat layout).
+
+The reference ships no sample codestreams, so every input is generated
+here, identically on any machine:
+
+* PRNG: splitmix64; one sub-stream per (image, tile, component, band) with
+  seed ^ (img << 40 | tile << 20 | comp << 8 | band).
+* uniform u in [0,1) from the top 53 bits; Laplacian(b) by the inverse CDF
+  x = -b*sgn(u-1/2)*ln(1-2|u-1/2|); Normal by Box-Muller in float64;
+  integers by rounding half away from zero.
+
+Band layout per tile-component (as the reference GPU "orig" buffer,
+decoder/cudainterface/src/cudainterface.cpp:674-703): at synthesis level
+lev (0 = coarsest) LL is at [0,lw)x[0,lh) (level 0 only), HL at
+[lw,nw)x[0,lh), LH at [0,lw)x[lh,nh), HH at [lw,nw)x[lh,nh).
+Subband ids: 0 = LL, 3*lev+1 = HL, 3*lev+2 = LH, 3*lev+3 = HH.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """First n outputs of splitmix64 started at `seed`."""
+    with np.errstate(over="ignore"):
+        i = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + i * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, n: int) -> np.ndarray:
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def laplace(seed: int, n: int, b: float) -> np.ndarray:
+    u = uniform(seed, n)
+    d = u - 0.5
+    a = np.minimum(2.0 * np.abs(d), 1.0 - 2.0 ** -53)
+    return -b * np.sign(d) * np.log1p(-a)
+
+
+def normal(seed: int, n: int, mu: float, sigma: float) -> np.ndarray:
+    u1 = uniform(seed, n)
+    u2 = uniform(seed ^ 0x5851F42D4C957F2D, n)
+    r = np.sqrt(-2.0 * np.log1p(-u1))
+    return mu + sigma * r * np.cos(2.0 * math.pi * u2)
+
+
+def round_half_away(x: np.ndarray) -> np.ndarray:
+    return np.where(x >= 0, np.floor(x + 0.5), -np.floor(-x + 0.5))
+
+
+def substream(seed: int, img: int, tile: int, comp: int, band: int) -> int:
+    return (seed ^ ((img << 40) | (tile << 20) | (comp << 8) | band)) & 0xFFFFFFFFFFFFFFFF
+
+
+def level_span(t0: int, n: int, levels: int, lev: int):
+    s = levels - 1 - lev
+    return -(-t0 // (1 << s)), -(-(t0 + n) // (1 << s))
+
+
+def band_rects(w: int, h: int, x0: int, y0: int, levels: int):
+    """[(band_id, decomposition_level, x, y, bw, bh)] in Mallat layout;
+    decomposition level 1 is the finest."""
+    rects = []
+    for lev in range(levels):
+        a0, a1 = level_span(x0, w, levels, lev)
+        b0, b1 = level_span(y0, h, levels, lev)
+        nw, nh = a1 - a0, b1 - b0
+        lw = (a1 + 1) // 2 - (a0 + 1) // 2
+        lh = (b1 + 1) // 2 - (b0 + 1) // 2
+        dl = levels - lev
+        if lev == 0:
+            rects.append((0, dl, 0, 0, lw, lh))
+        rects.append((3 * lev + 1, dl, lw, 0, nw - lw, lh))
+        rects.append((3 * lev + 2, dl, 0, lh, lw, nh - lh))
+        rects.append((3 * lev + 3, dl, lw, lh, nw - lw, nh - lh))
+    if levels == 0:
+        rects.append((0, 0, 0, 0, w, h))
+    return rects
+
+
+def tile_planes(w: int, h: int, levels: int, filt: str, ncomp: int, seed: int = 0, *,
+                x0: int = 0, y0: int = 0, img: int = 0, tile: int = 0,
+                ll53=(0, 4095), lap53: float = 6.0, ll97=(128.0, 40.0), lap97: float = 4.0,
+                signmag: bool = False) -> np.ndarray:
+    """Coefficient planes (ncomp, h, w).
+
+    5/3: int32; LL uniform integers in [ll53[0], ll53[1]], detail bands
+    round(Laplace(lap53)).
+    9/7: float32; LL ~ N(ll97), detail at decomposition level l ~
+    Laplace(lap97) * 2^-l.  With signmag=True the 9/7 values are rounded to
+    integers and stored as T1 sign-magnitude words (uint32).
+    """
+    if filt not in ("53", "97"):
+        raise ValueError(filt)
+    dtype = np.int32 if filt == "53" else np.float32
+    out = np.zeros((ncomp, h, w), dtype=dtype)
+    for c in range(ncomp):
+        for (bid, dl, bx, by, bw, bh) in band_rects(w, h, x0, y0, levels):
+            n = bw * bh
+            if n == 0:
+                continue
+            s = substream(seed, img, tile, c, bid)
+            if filt == "53":
+                if bid == 0:
+                    lo, hi = ll53
+                    v = np.floor(uniform(s, n) * (hi - lo + 1)) + lo
+                else:
+                    v = round_half_away(laplace(s, n, lap53))
+            else:
+                if bid == 0:
+                    v = normal(s, n, ll97[0], ll97[1])
+                else:
+                    v = laplace(s, n, lap97) * (2.0 ** -dl)
+            out[c, by:by + bh, bx:bx + bw] = v.reshape(bh, bw).astype(dtype)
+    if filt == "97" and signmag:
+        return to_signmag(round_half_away(out.astype(np.float64)).astype(np.int64))
+    return out
+
+
+def to_signmag(v: np.ndarray) -> np.ndarray:
+    """int -> T1 sign-magnitude words (bit 31 = sign)."""
+    v = np.asarray(v, dtype=np.int64)
+    mag = np.minimum(np.abs(v), 0x7FFFFFFF).astype(np.uint32)
+    return np.where(v < 0, mag | np.uint32(0x80000000), mag).astype(np.uint32)
+
+
+# --------------------------------------------------------------- configs --
+
+def config1(seed: int = 0):
+    """512x512 single component, 5 levels, 5/3 int32 (grey output)."""
+    return dict(w=512, h=512, levels=5, filt="53", ncomp=1,
+                planes=tile_planes(512, 512, 5, "53", 1, seed))
+
+
+def config2(seed: int = 0, w: int = 4096, h: int = 2160):
+    """4096x2160 3 components, 5 levels, 9/7 float32, ICT to 8-bit BGR."""
+    return dict(w=w, h=h, levels=5, filt="97", ncomp=3,
+                planes=tile_planes(w, h, 5, "97", 3, seed))
+
+
+def config3_params(seed: int = 0, ntiles: int = 256, lo: int = 64, hi: int = 1024):
+    """Tile parameters of the batched workload: sides U{lo..hi} (odd sizes
+    included), levels U{3..6}, 5/3 or 9/7 (p=1/2), 1 or 3 components."""
+    u = uniform(seed ^ 0xC3C3C3C3, ntiles * 5).reshape(ntiles, 5)
+    tiles = []
+    for i in range(ntiles):
+        w = lo + int(u[i, 0] * (hi - lo + 1))
+        h = lo + int(u[i, 1] * (hi - lo + 1))
+        levels = 3 + int(u[i, 2] * 4)
+        filt = "53" if u[i, 3] < 0.5 else "97"
+        ncomp = 1 if u[i, 4] < 0.5 else 3
+        tiles.append(dict(w=w, h=h, levels=levels, filt=filt, ncomp=ncomp, tile=i))
+    return tiles
+
+
+def config4_requests(seed: int, nreq: int, rate: float, lo: int = 256, hi: int = 8192):
+    """Poisson arrivals at `rate` req/s; square sides log-uniform in
+    [lo, hi]; 8 or 12 bits; 5/3 or 9/7; 3 components; 5 levels."""
+    u = uniform(seed ^ 0x4A4A4A4A, nreq * 4).reshape(nreq, 4)
+    t = np.cumsum(-np.log1p(-u[:, 0]) / rate)
+    reqs = []
+    for i in range(nreq):
+        side = int(round(math.exp(math.log(lo) + u[i, 1] * (math.log(hi) - math.log(lo)))))
+        reqs.append(dict(arrival=float(t[i]), w=side, h=side, bits=8 if u[i, 2] < 0.5 else 12,
+                         filt="53" if u[i, 3] < 0.5 else "97", ncomp=3, levels=5, img=i))
+    return reqs

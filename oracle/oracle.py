@@ -1,0 +1,213 @@
+"""ctypes wrapper of the CPU oracle (oracle/libj2k_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product.  Each function restates
+the reference CPU back end (see j2k_oracle.c for the file:line of every
+function it follows).
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import subprocess
+
+import numpy as np
+
+ORACLE_DIR = pathlib.Path(__file__).resolve().parent
+LIB_PATH = ORACLE_DIR / "libj2k_oracle.so"
+
+R53_REFERENCE = 0
+R53_TEXTBOOK = 1
+COEF_SIGNMAG = 0
+COEF_FLOAT = 1
+
+_LIB = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-B", "-C", str(ORACLE_DIR)], check=True,
+                   stdout=subprocess.DEVNULL)
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        P = ctypes.c_void_p
+        i = ctypes.c_int
+        L.or_idwt53.argtypes = [P, i, i, i, i, i, i, i, i]
+        L.or_idwt97.argtypes = [P, i, i, i, i, i, i, i, P, i]
+        L.or_idwt97_f64.argtypes = [P, i, i, i, i, i, i, P, i, P]
+        L.or_fdwt97_f64.argtypes = [P, i, i, i, i, i, P]
+        L.or_fdwt53.argtypes = [P, i, i, i, i, i, P]
+        L.or_53_line_textbook.argtypes = [P, i, i]
+        L.or_97_line.argtypes = [P, i, i]
+        L.or_h53_line_ref.argtypes = [P, P, P, i, i]
+        L.or_v53_ref.argtypes = [P, P, i, i, i]
+        L.or_rct_pack.argtypes = [P, i, P, P, P, i, i, i, i, i, i]
+        L.or_ict_pack.argtypes = [P, i, P, P, P, i, i, i, i, i, i]
+        L.or_merge_pack.argtypes = [P, i, P, i, i, i, i, i, i, i, i]
+        L.or_dc_shift.argtypes = [P, i, P, i, i, i, i, i]
+        L.or_ict_float.argtypes = [P, P, P, P, P, P, i, i]
+        L.or_pixel_bytes.argtypes = [i]
+        _LIB = L
+    return _LIB
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def idwt53(plane: np.ndarray, levels: int, *, x0=0, y0=0, toplevel=-1, mode=R53_REFERENCE):
+    """In-place-style 5/3 IDWT of one tile-component; returns a new array."""
+    a = np.ascontiguousarray(plane, dtype=np.int32).copy()
+    h, w = a.shape
+    rc = lib().or_idwt53(_p(a), w, w, h, x0, y0, levels, toplevel, mode)
+    assert rc == 0, rc
+    return a
+
+
+def idwt97(words: np.ndarray, levels: int, steps=None, *, x0=0, y0=0, toplevel=-1,
+           coef_kind=COEF_FLOAT):
+    """9/7 IDWT; words is a float32 or uint32 (sign-magnitude) plane."""
+    a = np.ascontiguousarray(words).view(np.uint32).copy()
+    h, w = a.shape
+    st = np.ones(3 * max(levels, 1) + 1, np.float32) if steps is None else np.asarray(steps, np.float32)
+    rc = lib().or_idwt97(_p(a), w, w, h, x0, y0, levels, toplevel, _p(st), coef_kind)
+    assert rc == 0, rc
+    return a.view(np.float32)
+
+
+def idwt97_f64(words: np.ndarray, levels: int, steps=None, *, x0=0, y0=0, coef_kind=COEF_FLOAT):
+    a = np.ascontiguousarray(words).view(np.uint32)
+    h, w = a.shape
+    st = np.ones(3 * max(levels, 1) + 1, np.float32) if steps is None else np.asarray(steps, np.float32)
+    out = np.zeros((h, w), np.float64)
+    rc = lib().or_idwt97_f64(_p(a), w, w, h, x0, y0, levels, _p(st), coef_kind, _p(out))
+    assert rc == 0, rc
+    return out
+
+
+def fdwt53(img: np.ndarray, levels: int, *, x0=0, y0=0):
+    a = np.ascontiguousarray(img, dtype=np.int32)
+    h, w = a.shape
+    out = np.zeros_like(a)
+    assert lib().or_fdwt53(_p(a), w, h, x0, y0, levels, _p(out)) == 0
+    return out
+
+
+def fdwt97_f64(img: np.ndarray, levels: int, *, x0=0, y0=0):
+    a = np.ascontiguousarray(img, dtype=np.float64)
+    h, w = a.shape
+    out = np.zeros_like(a)
+    assert lib().or_fdwt97_f64(_p(a), w, h, x0, y0, levels, _p(out)) == 0
+    return out
+
+
+def line53_textbook(x: np.ndarray, parity: int):
+    a = np.ascontiguousarray(x, dtype=np.int32).copy()
+    lib().or_53_line_textbook(_p(a), a.size, parity)
+    return a
+
+
+def line97(x: np.ndarray, parity: int):
+    a = np.ascontiguousarray(x, dtype=np.float32).copy()
+    lib().or_97_line(_p(a), a.size, parity)
+    return a
+
+
+def v53_ref(hres: np.ndarray, lh: int):
+    a = np.ascontiguousarray(hres, dtype=np.int32)
+    n, w = a.shape
+    out = np.zeros_like(a)
+    lib().or_v53_ref(_p(out), _p(a), lh, n - lh, w)
+    return out
+
+
+PIXEL_BYTES = {0: 4, 1: 6, 2: 3, 3: 4, 4: 8, 5: 6, 6: 3, 8: 8}
+
+
+def rct_pack(c0, c1, c2, bit_depth: int, fmt: int, check_out: int = -1):
+    c = [np.ascontiguousarray(x, dtype=np.int32) for x in (c0, c1, c2)]
+    h, w = c[0].shape
+    out = np.zeros((h, w * PIXEL_BYTES[fmt]), np.uint8)
+    rc = lib().or_rct_pack(_p(out), w, _p(c[0]), _p(c[1]), _p(c[2]), w, w, h, bit_depth, fmt,
+                           check_out)
+    assert rc == 0, rc
+    return out
+
+
+def ict_pack(c0, c1, c2, bit_depth: int, fmt: int, check_out: int = -1):
+    c = [np.ascontiguousarray(x, dtype=np.float32) for x in (c0, c1, c2)]
+    h, w = c[0].shape
+    out = np.zeros((h, w * PIXEL_BYTES[fmt]), np.uint8)
+    rc = lib().or_ict_pack(_p(out), w, _p(c[0]), _p(c[1]), _p(c[2]), w, w, h, bit_depth, fmt,
+                           check_out)
+    assert rc == 0, rc
+    return out
+
+
+def ict_float(c0, c1, c2, bit_depth: int):
+    c = [np.ascontiguousarray(x, dtype=np.float32) for x in (c0, c1, c2)]
+    r, g, b = (np.zeros_like(c[0]) for _ in range(3))
+    lib().or_ict_float(_p(r), _p(g), _p(b), _p(c[0]), _p(c[1]), _p(c[2]), c[0].size, bit_depth)
+    return r, g, b
+
+
+def merge_pack(comps, bit_depth: int, fmt: int, check_out: int = -1, is_float: bool = False):
+    dt = np.float32 if is_float else np.int32
+    arrs = [None if x is None else np.ascontiguousarray(x, dtype=dt) for x in comps]
+    first = next(a for a in arrs if a is not None)
+    h, w = first.shape
+    ptrs = (ctypes.c_void_p * 4)(*[None if a is None else a.ctypes.data for a in arrs] +
+                                 [None] * (4 - len(arrs)))
+    out = np.zeros((h, w * PIXEL_BYTES[fmt]), np.uint8)
+    rc = lib().or_merge_pack(_p(out), w, ptrs, len(arrs), w, w, h, bit_depth, fmt, check_out,
+                             int(is_float))
+    assert rc == 0, rc
+    return out
+
+
+def dc_shift(plane: np.ndarray, bit_depth: int, is_float: bool):
+    a = np.ascontiguousarray(plane).view(np.uint32)
+    h, w = a.shape
+    out = np.zeros((h, w), np.int32)
+    lib().or_dc_shift(_p(out), w, _p(a), w, w, h, bit_depth, int(is_float))
+    return out
+
+
+def decode_tile(planes: np.ndarray, *, levels: int, filt: str, out_format: int, bit_depth: int = 8,
+                check_out: int = -1, x0=0, y0=0, toplevel=-1, steps=None,
+                coef_kind=COEF_FLOAT, r53_mode=R53_REFERENCE):
+    """Reference CPU pipeline for one tile: IDWT per component, then
+    RCT/ICT (RGB formats), merge (*_COMPONENTS) or the planes (NONE).
+    Returns (recon planes list, packed output or None)."""
+    ncomp = planes.shape[0]
+    rgb = out_format in (0, 1, 2, 8)
+    recon = []
+    for c in range(ncomp):
+        if filt == "53":
+            recon.append(idwt53(planes[c], levels, x0=x0, y0=y0, toplevel=toplevel, mode=r53_mode))
+        else:
+            st = None if steps is None else steps[c]
+            recon.append(idwt97(planes[c], levels, st, x0=x0, y0=y0, toplevel=toplevel,
+                                coef_kind=coef_kind))
+    if toplevel >= 0 and toplevel < levels:
+        s = levels - toplevel
+        a0, a1 = -(-x0 // (1 << s)), -(-(x0 + planes.shape[2]) // (1 << s))
+        b0, b1 = -(-y0 // (1 << s)), -(-(y0 + planes.shape[1]) // (1 << s))
+        recon = [r[:b1 - b0, :a1 - a0] for r in recon]
+    if out_format == 7:
+        return recon, None
+    if rgb:
+        fn = ict_pack if filt == "97" else rct_pack
+        return recon, fn(recon[0], recon[1], recon[2], bit_depth, out_format, check_out)
+    if check_out >= 0:
+        comps = [recon[check_out]]
+        co = 0
+    else:
+        comps = recon
+        co = -1
+    return recon, merge_pack(comps, bit_depth, out_format, co, is_float=(filt == "97"))

@@ -1,0 +1,74 @@
+"""The C-ABI library loads and exports every function include/*.h declares
+(no GPU calls), and the ctypes structs match the C layout."""
+import ctypes
+import pathlib
+import re
+
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    names = []
+    for h in sorted((ROOT / "include").glob("*.h")):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;",
+                             text, flags=re.M):
+            name = m.group(1)
+            if name in ("if", "while", "sizeof", "return"):
+                continue
+            names.append(name)
+    return sorted(set(names))
+
+
+def test_header_declares_reference_entry_points():
+    names = declared_functions()
+    # decoder/cudainterface/include/gpulib.h:24-57
+    for ref in ["GpuInit", "GpuRemove", "MallocGpuBuf", "FreeGpuBuf", "CreateANewStream",
+                "DestroyStream", "GpuSync", "Gpu2DCopy", "Gpu2DCopyWithDiffPitch",
+                "GpuLinearMemCopy", "BMI_comp_idwt", "BMI_resolution_idwt", "BMI_tile_MCT",
+                "BMI_tile_ICT", "BMI_tile_Merge", "BMI_tile_One_Component", "BMI_Comp_DC_SHIFT"]:
+        assert ref in names, ref
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    lib = pkg.lib()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts(pkg):
+    # offsets that C computes for the ABI structs (x86-64 / amdgcn host)
+    assert ctypes.sizeof(pkg.GpuInfo) == 4 * 2 + 256 + 4 * 5
+    d = pkg.BmiTileDesc
+    assert d.coef.offset == 0
+    assert d.coef_pitch.offset == 32
+    assert d.steps.offset == 92
+    assert d.out.offset == 400
+    assert ctypes.sizeof(pkg.BmiTuning) == 16
+
+
+def test_error_strings_without_gpu(pkg):
+    lib = pkg.lib()
+    assert lib.BMI_error_string(0) == b"ok"
+    assert lib.BMI_error_string(-5) == b"unsupported mode or format"
+    lib.BMI_set_r53_mode(1)
+    assert lib.BMI_get_r53_mode() == 1
+    lib.BMI_set_r53_mode(0)
+    assert lib.BMI_get_r53_mode() == 0
+
+
+def test_plan_rejects_bad_descriptors_without_gpu(pkg):
+    """Validation happens on the host before any device call."""
+    bad = pkg.tile_desc(coef_ptrs=[0], coef_pitch=4, width=8, height=8, levels=9,
+                        method=pkg.W5X3, ncomp=1, out_format=pkg.FMT_NONE)
+    with pytest.raises(pkg.BmiError):
+        pkg.BatchPlan([bad])
+    rgb_one_comp = pkg.tile_desc(coef_ptrs=[0], coef_pitch=8, width=8, height=8, levels=2,
+                                 method=pkg.W5X3, ncomp=1, out_format=pkg.FMT_24BITS_RGB,
+                                 out_ptr=1, out_pitch_px=8)
+    with pytest.raises(pkg.BmiError):
+        pkg.BatchPlan([rgb_one_comp])

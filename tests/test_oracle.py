@@ -1,0 +1,287 @@
+"""Pins of the CPU oracle (no GPU).  Each check is chosen so a plausible
+mistake in the restatement (dropped term, wrong sign or index, swapped
+operand, wrong rounding) fails it; none retypes the oracle's own code."""
+import itertools
+
+import numpy as np
+import pytest
+
+RNG = np.random.default_rng(1234)
+
+
+# ------------------------------------------------------------------ 5/3 --
+
+def t800_53_line(y, i0):
+    """ITU-T T.800 F.3.7 1D_SR with F.3.8.1 1D_FILTR_5-3R, written from the
+    standard's formulas on an explicitly extended signal (PSE)."""
+    n = len(y)
+    if n == 1:
+        return [y[0] if i0 % 2 == 0 else y[0] // 2]
+    i1 = i0 + n
+    ext = 2
+    def pse(i):  # periodic symmetric extension index, T.800 eq. (F-4)
+        period = 2 * (i1 - i0 - 1)
+        j = (i - i0) % period
+        return i0 + min(j, period - j)
+    Y = {i: y[pse(i) - i0] for i in range(i0 - ext - 1, i1 + ext + 1)}
+    X = dict(Y)
+    for n2 in range((i0 - ext) // 2, (i1 + ext) // 2 + 1):
+        i = 2 * n2
+        if i0 - ext + 1 <= i <= i1 + ext - 2:
+            X[i] = Y[i] - ((Y[i - 1] + Y[i + 1] + 2) // 4)
+    for n2 in range((i0 - ext) // 2, (i1 + ext) // 2 + 1):
+        i = 2 * n2 + 1
+        if i0 - ext + 2 <= i <= i1 + ext - 3:
+            X[i] = Y[i] + ((X[i - 1] + X[i + 1]) // 2)
+    return [X[i] for i in range(i0, i1)]
+
+
+@pytest.mark.parametrize("n", list(range(1, 34)))
+def test_53_line_matches_t800_all_sizes(orc, n):
+    for i0 in (0, 1, 6, 7):
+        for _ in range(3):
+            y = RNG.integers(-300, 300, n).tolist()
+            got = orc.line53_textbook(np.array(y), i0 & 1).tolist()
+            assert got == t800_53_line(y, i0), (n, i0)
+
+
+@pytest.mark.parametrize("levels", [1, 2, 3, 5, 6])
+def test_53_textbook_reversible(orc, levels):
+    """forward 5/3 (T.800 F.4) then inverse is the identity, any size and
+    parity (T.800 F.3 reversibility)."""
+    for (w, h) in [(1, 1), (2, 3), (7, 5), (33, 17), (64, 64), (97, 131)]:
+        for (x0, y0) in [(0, 0), (1, 0), (0, 3), (5, 7)]:
+            img = RNG.integers(-2048, 2048, (h, w)).astype(np.int32)
+            coef = orc.fdwt53(img, levels, x0=x0, y0=y0)
+            back = orc.idwt53(coef, levels, x0=x0, y0=y0, mode=orc.R53_TEXTBOOK)
+            np.testing.assert_array_equal(back, img, err_msg=f"{w}x{h}@{x0},{y0} L{levels}")
+
+
+def test_53_constant_dc_gain_one(orc):
+    for mode in (orc.R53_REFERENCE, orc.R53_TEXTBOOK):
+        for (w, h, L) in [(64, 64, 5), (97, 131, 4), (512, 512, 5)]:
+            plane = np.zeros((h, w), np.int32)
+            import math
+            lw, lh = w, h
+            for _ in range(L):
+                lw, lh = (lw + 1) // 2, (lh + 1) // 2
+            plane[:lh, :lw] = 1234
+            out = orc.idwt53(plane, L, mode=mode)
+            assert (out == 1234).all()
+
+
+def closed_form_tail(hres, lh):
+    """Bottom rows of the reference's VDWT_R53_LINES from its hand trace:
+    m = n//2, P = H[max(m-2,0)]; even n: X[n-1] = P + X[n-2];
+    odd n: X[n-1] = L[m-1] - ((P+1)>>1), X[n-2] = P + ((X[n-1] + X[n-3])>>1)."""
+    n = hres.shape[0]
+    L, H = hres[:lh], hres[lh:]
+    m = n // 2
+    P = H[max(m - 2, 0)]
+    return m, P, L
+
+
+@pytest.mark.parametrize("n", list(range(2, 24)))
+def test_r53_quirk_closed_form(orc, n):
+    """Reference mode equals textbook mode except on the rows named by the
+    closed form of the bottom edge; those rows follow the closed form."""
+    w = 9
+    lh = (n + 1) // 2
+    hres = RNG.integers(-500, 500, (n, w)).astype(np.int32)
+    ref = orc.v53_ref(hres, lh)
+    # textbook vertical synthesis column by column
+    tb = np.zeros_like(hres)
+    for x in range(w):
+        inter = np.empty(n, np.int32)
+        inter[0::2] = hres[:lh, x]
+        inter[1::2] = hres[lh:, x]
+        tb[:, x] = orc.line53_textbook(inter, 0)
+    m, P, L = closed_form_tail(hres, lh)
+    if n % 2 == 0:
+        quirk_rows = [n - 1] if n >= 4 else []
+        exp_last = P + tb[n - 2]
+        np.testing.assert_array_equal(ref[n - 1], exp_last)
+    else:
+        quirk_rows = [n - 2, n - 1]
+        v1 = L[m - 1] - ((P + 1) >> 1)
+        v2 = P + ((v1 + tb[n - 3]) >> 1)
+        np.testing.assert_array_equal(ref[n - 1], v1)
+        np.testing.assert_array_equal(ref[n - 2], v2)
+    other = [r for r in range(n) if r not in quirk_rows]
+    np.testing.assert_array_equal(ref[other], tb[other])
+
+
+def test_h53_ref_equals_textbook_even_parity(orc):
+    """HDWT_R53_INT is textbook for parity 0 (its deviation is parity only)."""
+    for n in range(2, 40):
+        lw = (n + 1) // 2
+        lo = RNG.integers(-999, 999, lw).astype(np.int32)
+        hi = RNG.integers(-999, 999, n - lw).astype(np.int32)
+        inter = np.empty(n, np.int32)
+        inter[0::2], inter[1::2] = lo, hi
+        out = np.zeros(n, np.int32)
+        import ctypes
+        orc.lib().or_h53_line_ref(out.ctypes.data_as(ctypes.c_void_p), lo.ctypes.data_as(ctypes.c_void_p),
+                                  hi.ctypes.data_as(ctypes.c_void_p), lw, n - lw)
+        np.testing.assert_array_equal(out, orc.line53_textbook(inter, 0))
+
+
+def test_r53_reference_full_tile_differs_from_textbook(orc, workloads):
+    """On a multi-level tile the two modes must differ (the quirk is live),
+    and both must keep the DC-free details' energy bounded."""
+    planes = workloads.tile_planes(97, 131, 4, "53", 1, seed=3)
+    ref = orc.idwt53(planes[0], 4, mode=orc.R53_REFERENCE)
+    tb = orc.idwt53(planes[0], 4, mode=orc.R53_TEXTBOOK)
+    assert (ref != tb).any()
+    # the first rows are untouched by any level's bottom edge
+    np.testing.assert_array_equal(ref[:8], tb[:8])
+
+
+# ------------------------------------------------------------------ 9/7 --
+
+@pytest.mark.parametrize("levels", [1, 2, 3, 5])
+def test_97_f64_round_trip(orc, levels):
+    """FP64 forward (analysis) then the FP64 oracle's synthesis is the
+    identity: pins the lifting structure, constants, parities and the
+    reference gain convention (low x K, high x 2/K)."""
+    for (w, h) in [(1, 1), (2, 2), (3, 5), (6, 7), (33, 17), (97, 131)]:
+        for (x0, y0) in [(0, 0), (1, 1), (3, 0)]:
+            img = RNG.normal(0, 50, (h, w))
+            coef = orc.fdwt97_f64(img, levels, x0=x0, y0=y0)
+            back = orc.idwt97_f64(coef.astype(np.float32), levels, x0=x0, y0=y0)
+            # coefficients were rounded to float32 on the way in
+            assert np.abs(back - img).max() < 1e-3, (w, h, x0, y0)
+
+
+def test_97_float_close_to_f64(orc, workloads):
+    planes = workloads.tile_planes(256, 200, 5, "97", 1, seed=7)
+    f32 = orc.idwt97(planes[0], 5)
+    f64 = orc.idwt97_f64(planes[0], 5)
+    err = np.abs(f32.astype(np.float64) - f64)
+    assert err.max() < 2e-4 * max(1.0, np.abs(f64).max())
+    rms = np.sqrt((err ** 2).mean())
+    assert rms < 1e-4 * max(1.0, np.abs(f64).max())
+
+
+@pytest.mark.parametrize("n", [3, 5, 7, 9, 17, 31])
+def test_97_line_mirror_symmetry(orc, n):
+    """Odd length, parity 0: reversing the line keeps every sample's band
+    and, the operations being symmetric in (left, right), reverses the
+    output bit for bit.  Fails on a shifted neighbour or a wrong mirror."""
+    x = RNG.normal(0, 100, n).astype(np.float32)
+    a = orc.line97(x, 0)
+    b = orc.line97(x[::-1].copy(), 0)
+    np.testing.assert_array_equal(a, b[::-1])
+
+
+def test_signmag_equals_float_input(orc, workloads):
+    """Dequantising T1 sign-magnitude words equals feeding the same
+    integers as floats (FETCHPIXEL_INT, bmi_cpu_dwt_decoder.cpp:2287)."""
+    v = workloads.round_half_away(workloads.tile_planes(131, 97, 4, "97", 1, seed=9)[0].astype(np.float64))
+    sm = workloads.to_signmag(v.astype(np.int64))
+    steps = np.linspace(0.5, 2.0, 13).astype(np.float32)
+    a = orc.idwt97(sm, 4, steps, coef_kind=orc.COEF_SIGNMAG)
+    b = orc.idwt97(v.astype(np.float32), 4, steps, coef_kind=orc.COEF_FLOAT)
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_signmag_words(workloads):
+    v = np.array([0, 1, -1, 5, -5, 2 ** 30, -(2 ** 30)], np.int64)
+    w = workloads.to_signmag(v)
+    assert w.tolist() == [0, 1, 0x80000001, 5, 0x80000005, 2 ** 30, 0x80000000 | 2 ** 30]
+
+
+# --------------------------------------------------------------- colour --
+
+def forward_rct(R, G, B, bd):
+    off = 1 << (bd - 1)
+    r, g, b = R - off, G - off, B - off
+    return (r + 2 * g + b) >> 2, b - g, r - g
+
+
+def test_rct_exhaustive_8bit(orc):
+    """Forward RCT (T.800 G.2) then the oracle's inverse returns every 8-bit
+    triple exactly, with the B,G,R byte order of RGB_24BITS."""
+    v = np.arange(256, dtype=np.int32)
+    R, G, B = np.meshgrid(v, v, v, indexing="ij")
+    R, G, B = R.reshape(4096, 4096), G.reshape(4096, 4096), B.reshape(4096, 4096)
+    y0, y1, y2 = forward_rct(R, G, B, 8)
+    out = orc.rct_pack(y0, y1, y2, 8, 2).reshape(4096, 4096, 3)
+    np.testing.assert_array_equal(out[..., 0], B)
+    np.testing.assert_array_equal(out[..., 1], G)
+    np.testing.assert_array_equal(out[..., 2], R)
+
+
+def test_rct_12bit_into_48bit(orc):
+    R, G, B = (RNG.integers(0, 4096, (512, 512)).astype(np.int32) for _ in range(3))
+    y0, y1, y2 = forward_rct(R, G, B, 12)
+    out = orc.rct_pack(y0, y1, y2, 12, 1).view(np.uint16).reshape(512, 512, 3)
+    np.testing.assert_array_equal(out[..., 2], R << 4)
+    np.testing.assert_array_equal(out[..., 1], G << 4)
+    np.testing.assert_array_equal(out[..., 0], B << 4)
+
+
+def test_rct_floor_not_truncation(orc):
+    """(Y1+Y2)>>2 floors negative sums (the CPU's RRct_53), unlike the
+    reference CUDA kernel's /4 (bmi_dwt_gpu_kernel.h:1115)."""
+    y0 = np.array([[0]], np.int32)
+    y1 = np.array([[-1]], np.int32)
+    y2 = np.array([[-2]], np.int32)
+    out = orc.rct_pack(y0, y1, y2, 8, 2)
+    g = 0 - (-3 >> 2) + 128  # floor(-3/4) = -1 -> 129
+    assert out[0, 1] == g
+
+
+def test_ict_truncation_and_clamp_order(orc):
+    """Cb = Cr = 0: R = G = B = (int)clamp(Y + 2^(B-1)) on a 1/64 grid
+    around the clamp limits (truncation after the clamp, in float)."""
+    for bd in (8, 12):
+        top = 1 << bd
+        y = (np.arange(-(top + 2) * 64, (top + 2) * 64) / 64.0).astype(np.float32) - (1 << (bd - 1))
+        y = y.reshape(1, -1)
+        z = np.zeros_like(y)
+        fmt = 2 if bd == 8 else 1
+        out = orc.ict_pack(y, z, z, bd, fmt)
+        v = np.clip(y.astype(np.float64) + (1 << (bd - 1)), 0, top - 1)
+        exp = np.trunc(v).astype(np.int64)
+        if bd == 8:
+            got = out.reshape(-1, 3)
+        else:
+            got = out.view(np.uint16).reshape(-1, 3).astype(np.int64) >> 4
+        for ch in range(3):
+            np.testing.assert_array_equal(got[:, ch], exp[0])
+
+
+def test_ict_close_to_f64(orc):
+    c = [RNG.normal(0, 60, (64, 64)).astype(np.float32) for _ in range(3)]
+    r, g, b = orc.ict_float(*c, 8)
+    y, cb, cr = (x.astype(np.float64) for x in c)
+    np.testing.assert_allclose(r, y + 1.402 * cr + 128, atol=1e-3)
+    np.testing.assert_allclose(g, y - 0.34413 * cb - 0.71414 * cr + 128, atol=1e-3)
+    np.testing.assert_allclose(b, y + 1.772 * cb + 128, atol=1e-3)
+
+
+def test_merge_truncates_before_shift(orc):
+    """static_cast<int>(v) + 2^(B-1): -0.5 -> 0 + 128 (not floor -1)."""
+    v = np.array([[-0.5, -1.5, 0.7, 200.0, -300.0]], np.float32)
+    out = orc.merge_pack([v], 8, 6, 0, is_float=True).reshape(-1, 3)
+    assert out[:, 0].tolist() == [128, 127, 128, 255, 0]
+
+
+def test_merge_component_slots(orc):
+    a = np.full((1, 2), 10, np.int32)
+    b = np.full((1, 2), 20, np.int32)
+    c = np.full((1, 2), 30, np.int32)
+    d = np.full((1, 2), 40, np.int32)
+    out4 = orc.merge_pack([a, b, c, d], 8, 3).reshape(-1, 4)
+    # 4 components: byte 0 = Comp3, 1 = Comp2, 2 = Comp1, 3 = Comp0
+    assert out4[0].tolist() == [168, 158, 148, 138]
+    out3 = orc.merge_pack([a, b, c], 8, 6).reshape(-1, 3)
+    assert out3[0].tolist() == [158, 148, 138]
+
+
+def test_dc_shift(orc):
+    v = np.array([[-200, -128, 0, 127, 300]], np.int32)
+    assert orc.dc_shift(v, 8, False).tolist() == [[0, 0, 128, 255, 255]]
+    f = np.array([[-0.5, 1.9, -128.9]], np.float32)
+    assert orc.dc_shift(f, 8, True).tolist() == [[128, 129, 0]]
