@@ -366,51 +366,51 @@ __global__ void __launch_bounds__(1024) fused_persistent_kernel(const LevelJob *
                                                                 int *counters, int njobs,
                                                                 int total_waves)
 {
+    // All control flow below depends only on wave-uniform values
+    // (readfirstlane results and kernel arguments).  Every lane executes the
+    // atomics, lane 0 contributing 1 and the others 0, so no branch is
+    // lane-divergent and the compiler cannot split the wave across loop
+    // iterations around the queue operations.
     int *head = counters + njobs;
     int *exited = counters + njobs + 1;
     int *error_flag = counters + njobs + 2;
     const int lane = threadIdx.x & 63;
+    const int one = (lane == 0) ? 1 : 0;
     for (;;) {
-        int i = 0;
-        if (lane == 0) i = atomicAdd(head, 1);
-        i = __builtin_amdgcn_readfirstlane(__shfl(i, 0, 64));
+        const int i = __builtin_amdgcn_readfirstlane(atomicAdd(head, one));
         if (i >= n_items) break;
-        const WorkItem it = items[i];
-        const LevelJob &J = jobs[it.job];
-        if (lane == 0) {
-            for (int d = 0; d < J.ndeps; ++d) {
-                const int dj = J.deps[d];
-                const int need = jobs[dj].n_items;
-                int spins = 0;
-                while (__hip_atomic_load(&counters[dj], __ATOMIC_ACQUIRE,
-                                         __HIP_MEMORY_SCOPE_AGENT) < need) {
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++spins > kSpinLimit) {
-                        atomicOr(error_flag, 1);
-                        break;
-                    }
+        const int job = __builtin_amdgcn_readfirstlane(items[i].job);
+        const int local = __builtin_amdgcn_readfirstlane(items[i].local);
+        const LevelJob &J = jobs[job];
+        const int ndeps = J.ndeps;
+        for (int d = 0; d < ndeps; ++d) {
+            const int dj = J.deps[d];
+            const int need = jobs[dj].n_items;
+            for (int spins = 0;; ++spins) {
+                const int done = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
+                    &counters[dj], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+                if (done >= need) break;
+                const int err = __builtin_amdgcn_readfirstlane(
+                    __hip_atomic_load(error_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                if (err || spins > kSpinLimit) {
+                    atomicOr(error_flag, one);
+                    break;
                 }
+                __builtin_amdgcn_s_sleep(1);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        run_item(jobs, it, lane);
+        run_item(jobs, WorkItem{job, local}, lane);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (lane == 0)
-            __hip_atomic_fetch_add(&counters[it.job], 1, __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&counters[job], one, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
-    int last = 0;
-    if (lane == 0)
-        last = __hip_atomic_fetch_add(exited, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
-               total_waves - 1;
-    last = __shfl(last, 0, 64);
-    if (last) {
+    const int prev = __builtin_amdgcn_readfirstlane(
+        __hip_atomic_fetch_add(exited, one, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT));
+    if (prev == total_waves - 1) {
         for (int j = lane; j < njobs; j += 64)
             __hip_atomic_store(&counters[j], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (lane == 0) {
-            __hip_atomic_store(head, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(exited, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        __hip_atomic_store(head, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(exited, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     }
 }
