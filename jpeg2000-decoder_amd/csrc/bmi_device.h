@@ -44,6 +44,21 @@ __host__ __device__ inline int reflect(int c, int a0, int a1)
     return a0 + (i < n ? i : period - i);
 }
 
+// Global-address-space views of pointers that reach a kernel through
+// descriptors in memory.  Without the cast the compiler emits FLAT
+// accesses, which share a wait counter with LDS and scalar loads and
+// serialise every load behind an s_waitcnt vmcnt(0) lgkmcnt(0).
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T *gptr(const void *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ inline __attribute__((address_space(1))) T *gptr_w(void *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 // two's complement of a T1 sign-magnitude word (FETCHPIXEL_INT,
 // bmi_cpu_dwt_decoder.cpp:2287)
 __device__ inline int signmag_to_int(uint32_t w)
@@ -60,12 +75,14 @@ __device__ inline float lift97(float x, float c, float l, float r)
     return __fsub_rn(x, p);
 }
 
-// wave64 neighbour exchange: value of lane-1 / lane+1 (edge lanes get
-// their own value back; they are halo lanes and never stored)
-__device__ inline float wave_from_left(float v) { return __shfl_up(v, 1, 64); }
-__device__ inline float wave_from_right(float v) { return __shfl_down(v, 1, 64); }
-__device__ inline int wave_from_left(int v) { return __shfl_up(v, 1, 64); }
-__device__ inline int wave_from_right(int v) { return __shfl_down(v, 1, 64); }
+// wave64 neighbour exchange with DPP wave shifts (a VALU move, no LDS
+// round trip): wave_shr:1 gives each lane the value of lane-1, wave_shl:1
+// the value of lane+1.  Edge lanes keep their own value; they are halo
+// lanes and never stored.
+__device__ inline int wave_from_left(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+__device__ inline int wave_from_right(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+__device__ inline float wave_from_left(float v) { return __int_as_float(wave_from_left(__float_as_int(v))); }
+__device__ inline float wave_from_right(float v) { return __int_as_float(wave_from_right(__float_as_int(v))); }
 
 // LIFT, bmi_cpu_dwt.h:22
 __device__ inline int lift_depth(int v, int in_depth, int out_depth)
@@ -76,13 +93,14 @@ __device__ inline int lift_depth(int v, int in_depth, int out_depth)
 
 // ---- pixel packing (element layouts of codec_base.h:51-143: B,G,R[,A]) ----
 
-__device__ inline void put_pixel(uint8_t *p, int fmt, int R, int G, int B, int A)
+__device__ inline void put_pixel(uint8_t *pg, int fmt, int R, int G, int B, int A)
 {
+    auto p = gptr_w<uint8_t>(pg);
     switch (fmt) {
     case IMAGE_FOTMAT_32BITS_ARGB:
     case IMAGE_FOTMAT_32BITS_4COMPONENTS:
-        *(uint32_t *)p = (uint32_t)(B & 0xff) | ((uint32_t)(G & 0xff) << 8) |
-                         ((uint32_t)(R & 0xff) << 16) | ((uint32_t)(A & 0xff) << 24);
+        *gptr_w<uint32_t>(pg) = (uint32_t)(B & 0xff) | ((uint32_t)(G & 0xff) << 8) |
+                                ((uint32_t)(R & 0xff) << 16) | ((uint32_t)(A & 0xff) << 24);
         break;
     case IMAGE_FOTMAT_24BITS_RGB:
     case IMAGE_FOTMAT_24BITS_3COMPONENTS:
@@ -92,13 +110,13 @@ __device__ inline void put_pixel(uint8_t *p, int fmt, int R, int G, int B, int A
         break;
     case IMAGE_FOTMAT_48BITS_RGB:
     case IMAGE_FOTMAT_48BITS_3COMPONENTS: {
-        uint16_t *q = (uint16_t *)p;
+        auto q = gptr_w<uint16_t>(pg);
         q[0] = (uint16_t)B;
         q[1] = (uint16_t)G;
         q[2] = (uint16_t)R;
     } break;
     default: {  // 64-bit ARGB / 4 components
-        uint16_t *q = (uint16_t *)p;
+        auto q = gptr_w<uint16_t>(pg);
         q[0] = (uint16_t)B;
         q[1] = (uint16_t)G;
         q[2] = (uint16_t)R;

@@ -54,11 +54,11 @@ __device__ inline typename SampleT<FILT>::T load_sample(const void *base, size_t
                                                         float gain)
 {
     if constexpr (FILT == 97) {
-        uint32_t w = ((const uint32_t *)base)[idx];
+        uint32_t w = gptr<uint32_t>(base)[idx];
         float v = (kind == SRC_SIGNMAG) ? (float)signmag_to_int(w) : __uint_as_float(w);
         return __fmul_rn(v, gain);
     } else {
-        return ((const int *)base)[idx];
+        return gptr<int>(base)[idx];
     }
 }
 
@@ -105,7 +105,7 @@ __device__ inline void emit_pixel(const LevelJob &J, int row, int col,
         size_t idx = (size_t)(row - J.b0) * J.dst_pitch + (col - J.a0);
 #pragma unroll
         for (int c = 0; c < NC; ++c)
-            if (c < J.nc) ((T *)J.dst[c])[idx] = v[c];
+            if (c < J.nc) gptr_w<T>(J.dst[c])[idx] = v[c];
         return;
     }
     const int bd = J.bit_depth;
@@ -137,87 +137,250 @@ __device__ inline void emit_pixel(const LevelJob &J, int row, int col,
     put_pixel(p, J.fmt, R, G, B, A);
 }
 
+// Prefetch depth of the row-pair pipeline: loads for pair k+D are issued
+// while pair k is lifted, so each wave keeps D pairs of loads in flight
+// (deeper for single-component items, which need fewer registers).
+template <int NC>
+struct Prefetch {
+    static constexpr int D = (NC == 1) ? 4 : 2;
+};
+
+// convert a raw 32-bit word to a sample: sign-magnitude or float words
+// times the folded gain (9/7), two's complement (5/3); select, no branch
+template <int FILT>
+__device__ inline typename SampleT<FILT>::T to_sample(uint32_t w, int is_float, float gain)
+{
+    if constexpr (FILT == 97) {
+        const float v = is_float ? __uint_as_float(w) : (float)signmag_to_int(w);
+        return __fmul_rn(v, gain);
+    } else {
+        return (int)w;
+    }
+}
+
+// Degenerate levels (a single row or a single column): no prefetching.
+template <int FILT, int NC>
+__device__ __attribute__((always_inline)) inline void process_item_small(const LevelJob &__restrict__ J,
+                                                             int strip, int seg, int lane);
+
 // One work item: strip `strip`, row segment `seg` of level job J, run by
 // one wave64 (lane = 0..63).
 template <int FILT, int NC>
-__device__ void process_item(const LevelJob &J, int strip, int seg, int lane)
+__device__ __attribute__((always_inline)) inline void process_item(
+    const LevelJob &__restrict__ J, int strip, int seg, int lane)
 {
     using T = typename SampleT<FILT>::T;
     constexpr int HP = (FILT == 97) ? 2 : 1;  // halo pairs on each side
     constexpr int OWP = 64 - 2 * HP;          // output pairs per strip
+    constexpr int D = Prefetch<NC>::D;
 
     const int a0 = J.a0, a1 = J.a1, b0 = J.b0, b1 = J.b1;
-    const bool single_col = (a1 - a0) == 1;
-    const bool single_row = (b1 - b0) == 1;
+    if ((a1 - a0) == 1 || (b1 - b0) == 1) {
+        process_item_small<FILT, NC>(J, strip, seg, lane);
+        return;
+    }
+    // every field the loop needs, hoisted out of the descriptor once
+    const int lw = J.lw, lh = J.lh, rl0 = J.rl0, rh0 = J.rh0;
+    const int ll_pitch = J.ll_pitch, det_pitch = J.det_pitch;
+    const int ll_float = J.ll_kind == SRC_FLOAT, det_float = J.det_kind == SRC_FLOAT;
+    const int seg_pairs = J.seg_pairs, quirk = J.quirk;
+    const __attribute__((address_space(1))) uint32_t *llp[NC];
+    const __attribute__((address_space(1))) uint32_t *detp[NC];
+    float gn[NC][4];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        llp[c] = gptr<uint32_t>(J.ll[c]);
+        detp[c] = gptr<uint32_t>(J.det[c]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) gn[c][j] = J.gain[c][j];
+    }
 
     // absolute columns of this lane (even = low sample, odd = high sample)
     const int wpair0 = ((a0 & ~1) >> 1) + strip * OWP - HP;
     const int ce = 2 * (wpair0 + lane), co = ce + 1;
     const bool lane_out = (lane >= HP) && (lane < 64 - HP);
-    int lo_idx = 0, hi_idx = 0;
-    if (!single_col) {
-        const int rce = reflect(ce, a0, a1), rco = reflect(co, a0, a1);
-        lo_idx = (rce >> 1) - J.cl0;
-        hi_idx = (rco >> 1) - J.ch0;
-    }
-    const bool col_is_low = single_col && !(a0 & 1);
+    const int lo_idx = (reflect(ce, a0, a1) >> 1) - J.cl0;
+    const int hi_off = lw + (reflect(co, a0, a1) >> 1) - J.ch0;
 
     // row pairs of this segment
     const int K0 = (b0 & ~1) >> 1;
     const int Klast = (b1 - 1) >> 1;
-    const int kout0 = K0 + seg * J.seg_pairs;
-    const int kout1 = min(Klast, kout0 + J.seg_pairs - 1);
+    const int kout0 = K0 + seg * seg_pairs;
+    const int kout1 = min(Klast, kout0 + seg_pairs - 1);
 
-    // load and horizontally synthesise the two rows of pair k for component
-    // c: (tl, th) = low row at columns (ce, co), (bl, bh) = high row
-    auto load_rows = [&](int k, int c, T &tl, T &th, T &bl, T &bh) {
-        int rre, rro;
-        if (!single_row) {
-            rre = reflect(2 * k, b0, b1);
-            rro = reflect(2 * k + 1, b0, b1);
-        } else {
-            rre = rro = b0;
-        }
-        const int r = (rre >> 1) - J.rl0;          // row in the low half
-        const int q = (rro >> 1) - J.rh0 + J.lh;   // plane row of the high half
-        const float *g = J.gain[c];
-        const bool has_top = !single_row || !(b0 & 1);
-        const bool has_bot = !single_row || (b0 & 1);
-        tl = th = bl = bh = (T)0;
-        if (!single_col) {
-            if (has_top) {
-                tl = load_sample<FILT>(J.ll[c], (size_t)r * J.ll_pitch + lo_idx, J.ll_kind, g[0]);
-                th = load_sample<FILT>(J.det[c], (size_t)r * J.det_pitch + J.lw + hi_idx,
-                                       J.det_kind, g[1]);
-            }
-            if (has_bot) {
-                bl = load_sample<FILT>(J.det[c], (size_t)q * J.det_pitch + lo_idx, J.det_kind,
-                                       g[2]);
-                bh = load_sample<FILT>(J.det[c], (size_t)q * J.det_pitch + J.lw + hi_idx,
-                                       J.det_kind, g[3]);
-            }
-            hlift<FILT>(tl, th);
-            hlift<FILT>(bl, bh);
-        } else {
-            if (col_is_low) {
-                if (has_top)
-                    tl = load_sample<FILT>(J.ll[c], (size_t)r * J.ll_pitch, J.ll_kind, g[0]);
-                if (has_bot)
-                    bl = load_sample<FILT>(J.det[c], (size_t)q * J.det_pitch, J.det_kind, g[2]);
-            } else {
-                if (has_top)
-                    tl = half_sample<FILT>(load_sample<FILT>(
-                        J.det[c], (size_t)r * J.det_pitch + J.lw, J.det_kind, g[1]));
-                if (has_bot)
-                    bl = half_sample<FILT>(load_sample<FILT>(
-                        J.det[c], (size_t)q * J.det_pitch + J.lw, J.det_kind, g[3]));
-            }
-            th = tl;
-            bh = bl;
+    // raw words of pair k: [comp][top-low, top-high, bottom-low, bottom-high]
+    auto fetch = [&](int k, uint32_t (&r)[NC][4]) {
+        const int rre = reflect(2 * k, b0, b1), rro = reflect(2 * k + 1, b0, b1);
+        const size_t rt = (size_t)((rre >> 1) - rl0);
+        const size_t q = (size_t)((rro >> 1) - rh0 + lh);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            r[c][0] = llp[c][rt * ll_pitch + lo_idx];
+            r[c][1] = detp[c][rt * det_pitch + hi_off];
+            r[c][2] = detp[c][q * det_pitch + lo_idx];
+            r[c][3] = detp[c][q * det_pitch + hi_off];
         }
     };
 
-    // ve = values at (row, ce), vo = values at (row, co), NC components
+    auto emit_row = [&](int row, const T *ve, const T *vo) {
+        if (!lane_out || row < b0 || row >= b1) return;
+        if (ce >= a0 && ce < a1) emit_pixel<FILT, NC>(J, row, ce, ve);
+        if (co >= a0 && co < a1) emit_pixel<FILT, NC>(J, row, co, vo);
+    };
+
+    // vertical pipeline state per component and column (0 = ce, 1 = co)
+    // 9/7: s0 = H_{k-1}, s1 = e3_{k-1}, s2 = o4_{k-2}, s3 = e5_{k-2}
+    // 5/3: s0 = H_{k-1}, s1 = e1_{k-1}, s2 = L_{k-1}, s3 = H_{k-2}
+    T s0[NC][2], s1[NC][2], s2[NC][2], s3[NC][2];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) s0[c][e] = s1[c][e] = s2[c][e] = s3[c][e] = (T)0;
+
+    const int nh = b1 - b0;
+    // iteration at which the reference's VDWT_R53_LINES tail applies:
+    // local pair nh/2 (bmi_cpu_dwt_decoder.cpp:3359-3378)
+    const int kq = (FILT == 53 && quirk) ? (K0 + (nh >> 1)) : -0x7fffffff;
+    const bool nh_even = !(nh & 1);
+    constexpr int LAG = (FILT == 97) ? 2 : 1;  // pair k in, pair k-LAG out
+    const int ks = kout0 - ((FILT == 97) ? 2 : (quirk ? 2 : 1));
+    const int ke = kout1 + LAG;
+
+    // one pipeline step with pair k
+    auto consume = [&](int k, const uint32_t (&raw)[NC][4]) {
+        T evn[2][NC], odd[2][NC];  // [column][component]
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            T tl = to_sample<FILT>(raw[c][0], ll_float, gn[c][0]);
+            T th = to_sample<FILT>(raw[c][1], det_float, gn[c][1]);
+            T bl = to_sample<FILT>(raw[c][2], det_float, gn[c][2]);
+            T bh = to_sample<FILT>(raw[c][3], det_float, gn[c][3]);
+            hlift<FILT>(tl, th);
+            hlift<FILT>(bl, bh);
+            const T Lk[2] = {tl, th}, Hk[2] = {bl, bh};
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                if constexpr (FILT == 97) {
+                    const float e3 = lift97(Lk[e], kDelta, s0[c][e], Hk[e]);
+                    const float o4 = lift97(s0[c][e], kGamma, s1[c][e], e3);
+                    const float e5 = lift97(s1[c][e], kBeta, s2[c][e], o4);
+                    const float o6 = lift97(s2[c][e], kAlpha, s3[c][e], e5);
+                    evn[e][c] = s3[c][e];
+                    odd[e][c] = o6;
+                    s0[c][e] = Hk[e];
+                    s1[c][e] = e3;
+                    s2[c][e] = o4;
+                    s3[c][e] = e5;
+                } else {
+                    int e1 = Lk[e] - ((s0[c][e] + Hk[e] + 2) >> 2);
+                    int o2 = s0[c][e] + ((s1[c][e] + e1) >> 1);
+                    if (k == kq) {
+                        if (nh_even) {
+                            // last odd row uses the stale high row H_{m-2}
+                            o2 = Hk[e] + s1[c][e];
+                        } else {
+                            // last two rows use L_{m-1} and H_{m-2}
+                            e1 = s2[c][e] - ((s3[c][e] + s3[c][e] + 2) >> 2);
+                            o2 = s3[c][e] + ((s1[c][e] + e1) >> 1);
+                        }
+                    }
+                    evn[e][c] = s1[c][e];
+                    odd[e][c] = o2;
+                    s3[c][e] = s0[c][e];
+                    s0[c][e] = Hk[e];
+                    s2[c][e] = Lk[e];
+                    s1[c][e] = e1;
+                }
+            }
+        }
+        const int ko = k - LAG;
+        if (ko >= kout0 && ko <= kout1) {
+            emit_row(2 * ko, evn[0], evn[1]);
+            emit_row(2 * ko + 1, odd[0], odd[1]);
+        }
+    };
+
+    // software pipeline, D pairs of loads in flight; loads are issued
+    // unconditionally (rows past the segment end are clamped to it and the
+    // pipeline steps that use them emit nothing)
+    uint32_t ring[D][NC][4];
+#pragma unroll
+    for (int d = 0; d < D; ++d) fetch(min(ks + d, ke), ring[d]);
+    const int rounds = (ke - ks + D) / D;
+    for (int it = 0; it < rounds; ++it) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int k = ks + it * D + d;
+            uint32_t cur[NC][4];
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) cur[c][j] = ring[d][c][j];
+            fetch(min(k + D, ke), ring[d]);
+            consume(k, cur);
+        }
+    }
+}
+
+template <int FILT, int NC>
+__device__ __attribute__((always_inline)) inline void process_item_small(const LevelJob &__restrict__ J,
+                                                             int strip, int seg, int lane)
+{
+    using T = typename SampleT<FILT>::T;
+    constexpr int HP = (FILT == 97) ? 2 : 1;
+    constexpr int OWP = 64 - 2 * HP;
+    const int a0 = J.a0, a1 = J.a1, b0 = J.b0, b1 = J.b1;
+    const bool single_col = (a1 - a0) == 1;
+    const bool single_row = (b1 - b0) == 1;
+    const int nc = J.nc;
+    const int wpair0 = ((a0 & ~1) >> 1) + strip * OWP - HP;
+    const int ce = 2 * (wpair0 + lane), co = ce + 1;
+    const bool lane_out = (lane >= HP) && (lane < 64 - HP);
+    int lo_idx = 0, hi_idx = 0;
+    if (!single_col) {
+        lo_idx = (reflect(ce, a0, a1) >> 1) - J.cl0;
+        hi_idx = (reflect(co, a0, a1) >> 1) - J.ch0;
+    }
+    const bool col_is_low = single_col && !(a0 & 1);
+    const int hi_off = single_col ? J.lw : J.lw + hi_idx;
+    const int K0 = (b0 & ~1) >> 1;
+    const int Klast = (b1 - 1) >> 1;
+    const int kout0 = K0 + seg * J.seg_pairs;
+    const int kout1 = min(Klast, kout0 + J.seg_pairs - 1);
+    const bool has_top = !single_row || !(b0 & 1);
+    const bool has_bot = !single_row || (b0 & 1);
+    const bool load_lo = !single_col || col_is_low;
+    const bool load_hi = !single_col || !col_is_low;
+
+    auto load_rows = [&](int k, int c, T &tl, T &th, T &bl, T &bh) {
+        int rre = b0, rro = b0;
+        if (!single_row) {
+            rre = reflect(2 * k, b0, b1);
+            rro = reflect(2 * k + 1, b0, b1);
+        }
+        const size_t rt = (size_t)((rre >> 1) - J.rl0);
+        const size_t q = (size_t)((rro >> 1) - J.rh0 + J.lh);
+        const float *g = J.gain[c];
+        tl = th = bl = bh = (T)0;
+        if (has_top) {
+            if (load_lo) tl = load_sample<FILT>(J.ll[c], rt * J.ll_pitch + lo_idx, J.ll_kind, g[0]);
+            if (load_hi) th = load_sample<FILT>(J.det[c], rt * J.det_pitch + hi_off, J.det_kind, g[1]);
+        }
+        if (has_bot) {
+            if (load_lo) bl = load_sample<FILT>(J.det[c], q * J.det_pitch + lo_idx, J.det_kind, g[2]);
+            if (load_hi) bh = load_sample<FILT>(J.det[c], q * J.det_pitch + hi_off, J.det_kind, g[3]);
+        }
+        if (!single_col) {
+            hlift<FILT>(tl, th);
+            hlift<FILT>(bl, bh);
+        } else if (col_is_low) {
+            th = tl;
+            bh = bl;
+        } else {
+            tl = th = half_sample<FILT>(th);
+            bl = bh = half_sample<FILT>(bh);
+        }
+    };
     auto emit_row = [&](int row, const T *ve, const T *vo) {
         if (!lane_out || row < b0 || row >= b1) return;
         if (ce >= a0 && ce < a1) emit_pixel<FILT, NC>(J, row, ce, ve);
@@ -229,7 +392,7 @@ __device__ void process_item(const LevelJob &J, int strip, int seg, int lane)
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             T tl = 0, th = 0, bl = 0, bh = 0;
-            if (c < J.nc) load_rows(b0 >> 1, c, tl, th, bl, bh);
+            if (c < nc) load_rows(b0 >> 1, c, tl, th, bl, bh);
             if (!(b0 & 1)) {
                 ve[c] = tl;
                 vo[c] = th;
@@ -241,97 +404,68 @@ __device__ void process_item(const LevelJob &J, int strip, int seg, int lane)
         emit_row(b0, ve, vo);
         return;
     }
-
-    if constexpr (FILT == 97) {
-        // per component and column (0 = ce, 1 = co): H_{k-1}, e3_{k-1},
-        // o4_{k-2}, e5_{k-2}; pair k in, pair k-2 out
-        float Hp[NC][2], e3p[NC][2], o4pp[NC][2], e5pp[NC][2];
+    // single column, several rows: plain (unpipelined) vertical lifting
+    T s0[NC][2], s1[NC][2], s2[NC][2], s3[NC][2];
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
+    for (int c = 0; c < NC; ++c)
 #pragma unroll
-            for (int e = 0; e < 2; ++e) Hp[c][e] = e3p[c][e] = o4pp[c][e] = e5pp[c][e] = 0.0f;
-        const int ks = kout0 - 2, ke = kout1 + 2;
-        for (int k = ks; k <= ke; ++k) {
-            float evn[2][NC], odd[2][NC];  // [column][component]
+        for (int e = 0; e < 2; ++e) s0[c][e] = s1[c][e] = s2[c][e] = s3[c][e] = (T)0;
+    const int nh = b1 - b0;
+    const int kq = (FILT == 53 && J.quirk) ? (K0 + (nh >> 1)) : -0x7fffffff;
+    const bool nh_even = !(nh & 1);
+    constexpr int LAG = (FILT == 97) ? 2 : 1;
+    const int ks = kout0 - ((FILT == 97) ? 2 : (J.quirk ? 2 : 1));
+    const int ke = kout1 + LAG;
+    for (int k = ks; k <= ke; ++k) {
+        T evn[2][NC], odd[2][NC];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                float tl = 0.f, th = 0.f, bl = 0.f, bh = 0.f;
-                if (c < J.nc) load_rows(k, c, tl, th, bl, bh);
-                const float Lk[2] = {tl, th}, Hk[2] = {bl, bh};
+        for (int c = 0; c < NC; ++c) {
+            T tl = 0, th = 0, bl = 0, bh = 0;
+            if (c < nc) load_rows(k, c, tl, th, bl, bh);
+            const T Lk[2] = {tl, th}, Hk[2] = {bl, bh};
 #pragma unroll
-                for (int e = 0; e < 2; ++e) {
-                    const float e3 = lift97(Lk[e], kDelta, Hp[c][e], Hk[e]);
-                    const float o4 = lift97(Hp[c][e], kGamma, e3p[c][e], e3);
-                    const float e5 = lift97(e3p[c][e], kBeta, o4pp[c][e], o4);
-                    const float o6 = lift97(o4pp[c][e], kAlpha, e5pp[c][e], e5);
-                    evn[e][c] = e5pp[c][e];
+            for (int e = 0; e < 2; ++e) {
+                if constexpr (FILT == 97) {
+                    const float e3 = lift97(Lk[e], kDelta, s0[c][e], Hk[e]);
+                    const float o4 = lift97(s0[c][e], kGamma, s1[c][e], e3);
+                    const float e5 = lift97(s1[c][e], kBeta, s2[c][e], o4);
+                    const float o6 = lift97(s2[c][e], kAlpha, s3[c][e], e5);
+                    evn[e][c] = s3[c][e];
                     odd[e][c] = o6;
-                    Hp[c][e] = Hk[e];
-                    e3p[c][e] = e3;
-                    o4pp[c][e] = o4;
-                    e5pp[c][e] = e5;
-                }
-            }
-            const int ko = k - 2;
-            if (ko >= kout0) {
-                emit_row(2 * ko, evn[0], evn[1]);
-                emit_row(2 * ko + 1, odd[0], odd[1]);
-            }
-        }
-    } else {
-        // per component and column: H_{k-1}, e1_{k-1}, plus L_{k-1} and
-        // H_{k-2} for the reference bottom-edge quirk; pair k in, k-1 out
-        int Hp[NC][2], e1p[NC][2], Lp[NC][2], Hpp[NC][2];
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-#pragma unroll
-            for (int e = 0; e < 2; ++e) Hp[c][e] = e1p[c][e] = Lp[c][e] = Hpp[c][e] = 0;
-        const int nh = b1 - b0;
-        // iteration at which the reference's VDWT_R53_LINES tail applies:
-        // local pair nh/2 (bmi_cpu_dwt_decoder.cpp:3359-3378)
-        const int kq = J.quirk ? (K0 + (nh >> 1)) : -0x7fffffff;
-        const bool nh_even = !(nh & 1);
-        const int ks = kout0 - (J.quirk ? 2 : 1), ke = kout1 + 1;
-        for (int k = ks; k <= ke; ++k) {
-            int evn[2][NC], odd[2][NC];
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                int tl = 0, th = 0, bl = 0, bh = 0;
-                if (c < J.nc) load_rows(k, c, tl, th, bl, bh);
-                const int Lk[2] = {tl, th}, Hk[2] = {bl, bh};
-#pragma unroll
-                for (int e = 0; e < 2; ++e) {
-                    int e1 = Lk[e] - ((Hp[c][e] + Hk[e] + 2) >> 2);
-                    int o2 = Hp[c][e] + ((e1p[c][e] + e1) >> 1);
+                    s0[c][e] = Hk[e];
+                    s1[c][e] = e3;
+                    s2[c][e] = o4;
+                    s3[c][e] = e5;
+                } else {
+                    int e1 = Lk[e] - ((s0[c][e] + Hk[e] + 2) >> 2);
+                    int o2 = s0[c][e] + ((s1[c][e] + e1) >> 1);
                     if (k == kq) {
                         if (nh_even) {
-                            // last odd row uses the stale high row H_{m-2}
-                            o2 = Hk[e] + e1p[c][e];
+                            o2 = Hk[e] + s1[c][e];
                         } else {
-                            // last two rows use L_{m-1} and H_{m-2}
-                            e1 = Lp[c][e] - ((Hpp[c][e] + Hpp[c][e] + 2) >> 2);
-                            o2 = Hpp[c][e] + ((e1p[c][e] + e1) >> 1);
+                            e1 = s2[c][e] - ((s3[c][e] + s3[c][e] + 2) >> 2);
+                            o2 = s3[c][e] + ((s1[c][e] + e1) >> 1);
                         }
                     }
-                    evn[e][c] = e1p[c][e];
+                    evn[e][c] = s1[c][e];
                     odd[e][c] = o2;
-                    Hpp[c][e] = Hp[c][e];
-                    Hp[c][e] = Hk[e];
-                    Lp[c][e] = Lk[e];
-                    e1p[c][e] = e1;
+                    s3[c][e] = s0[c][e];
+                    s0[c][e] = Hk[e];
+                    s2[c][e] = Lk[e];
+                    s1[c][e] = e1;
                 }
             }
-            const int ko = k - 1;
-            if (ko >= kout0) {
-                emit_row(2 * ko, evn[0], evn[1]);
-                emit_row(2 * ko + 1, odd[0], odd[1]);
-            }
+        }
+        const int ko = k - LAG;
+        if (ko >= kout0) {
+            emit_row(2 * ko, evn[0], evn[1]);
+            emit_row(2 * ko + 1, odd[0], odd[1]);
         }
     }
 }
 
 template <int FILT>
-__device__ inline void dispatch_nc(const LevelJob &J, int strip, int seg, int lane)
+__device__ __attribute__((always_inline)) inline void dispatch_nc(const LevelJob &__restrict__ J, int strip, int seg, int lane)
 {
     if (J.nc == 1)
         process_item<FILT, 1>(J, strip, seg, lane);
@@ -341,9 +475,9 @@ __device__ inline void dispatch_nc(const LevelJob &J, int strip, int seg, int la
         process_item<FILT, 4>(J, strip, seg, lane);
 }
 
-__device__ inline void run_item(const LevelJob *jobs, const WorkItem &it, int lane)
+__device__ __attribute__((always_inline)) inline void run_item(const LevelJob *__restrict__ jobs, const WorkItem &it, int lane)
 {
-    const LevelJob &J = jobs[it.job];
+    const LevelJob &__restrict__ J = jobs[it.job];
     const int strip = it.local % J.nstrips;
     const int seg = it.local / J.nstrips;
     if (J.filt == 97)
@@ -361,7 +495,7 @@ constexpr int kSpinLimit = 1 << 22;
 // last wave to leave zeroes the counts, head and exit count, so the next
 // launch (e.g. a graph replay) starts from a clean queue without a
 // separate reset kernel.
-__global__ void __launch_bounds__(1024) fused_persistent_kernel(const LevelJob *jobs,
+__global__ void __launch_bounds__(256) fused_persistent_kernel(const LevelJob *jobs,
                                                                 const WorkItem *items, int n_items,
                                                                 int *counters, int njobs,
                                                                 int total_waves)
@@ -388,7 +522,7 @@ __global__ void __launch_bounds__(1024) fused_persistent_kernel(const LevelJob *
             const int need = jobs[dj].n_items;
             for (int spins = 0;; ++spins) {
                 const int done = __builtin_amdgcn_readfirstlane(__hip_atomic_load(
-                    &counters[dj], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+                    &counters[dj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
                 if (done >= need) break;
                 const int err = __builtin_amdgcn_readfirstlane(
                     __hip_atomic_load(error_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
@@ -415,16 +549,21 @@ __global__ void __launch_bounds__(1024) fused_persistent_kernel(const LevelJob *
     }
 }
 
-// level-step mode: one launch per dependency step, no waiting
-__global__ void __launch_bounds__(1024) fused_step_kernel(const LevelJob *jobs,
-                                                          const WorkItem *items, int first,
-                                                          int count)
+// level-step mode: one launch per dependency step and item variant (no
+// waiting); each variant gets its own register budget
+template <int FILT, int NC>
+__global__ void __launch_bounds__(256) fused_step_kernel(const LevelJob *__restrict__ jobs,
+                                                         const WorkItem *__restrict__ items,
+                                                         int first, int count)
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int i = blockIdx.x * (blockDim.x >> 6) + wave;
     if (i >= count) return;
-    run_item(jobs, items[first + i], lane);
+    const int job = __builtin_amdgcn_readfirstlane(items[first + i].job);
+    const int local = __builtin_amdgcn_readfirstlane(items[first + i].local);
+    const LevelJob &__restrict__ J = jobs[job];
+    process_item<FILT, NC>(J, local % J.nstrips, local / J.nstrips, lane);
 }
 
 }  // namespace bmi
@@ -440,6 +579,10 @@ struct BmiBatchPlan {
     std::vector<LevelJob> jobs;
     std::vector<WorkItem> items;
     std::vector<int> step_first, step_count;  // level-step mode
+    struct Launch {
+        int first, count, variant;
+    };
+    std::vector<Launch> launches;  // level-step mode: per step and variant
     std::vector<int> job_tile;                // tile of each job
     BmiTuning tuning;
     LevelJob *d_jobs = nullptr;
@@ -458,6 +601,13 @@ struct BmiBatchPlan {
 };
 
 namespace {
+
+// kernel variant of a job: filter x lockstep width class
+int variant_of(const LevelJob &J)
+{
+    const int ncc = J.nc == 1 ? 0 : (J.nc <= 3 ? 1 : 2);
+    return (J.filt == 97 ? 3 : 0) + ncc;
+}
 
 int out_kind_for(const BmiTileDesc &t)
 {
@@ -548,7 +698,7 @@ extern "C" int BMI_batch_plan_create(BmiBatchPlan **out_plan, const BmiTileDesc 
     P->tuning.waves_per_group = (tuning && tuning->waves_per_group > 0) ? tuning->waves_per_group : 4;
     P->tuning.groups_per_cu = (tuning && tuning->groups_per_cu > 0) ? tuning->groups_per_cu : 8;
     P->tuning.persistent = tuning ? tuning->persistent : 1;
-    if (P->tuning.waves_per_group > 16) P->tuning.waves_per_group = 16;
+    if (P->tuning.waves_per_group > 4) P->tuning.waves_per_group = 4;
     if (hipGetDevice(&P->device) != hipSuccess) P->device = 0;
     {
         int cus = 0;
@@ -722,10 +872,28 @@ extern "C" int BMI_batch_plan_create(BmiBatchPlan **out_plan, const BmiTileDesc 
                 }
                 // a merge that checks out one component shows it as grey
                 if (ok == OUT_MERGE && t.check_out >= 0) J.check_out = 0;
+                // the kernel loads every lockstep slot: pad unused ones with
+                // component slot 0 (never emitted: slots >= nc are ignored)
+                for (int i = ncomps; i < 4; ++i) {
+                    J.comp[i] = J.comp[0];
+                    J.ll[i] = J.ll[0];
+                    J.det[i] = J.det[0];
+                    for (int g = 0; g < 4; ++g) J.gain[i][g] = J.gain[0][g];
+                }
                 push_job(J, -1);
             }
         }
         P->step_count[step] = (int)P->items.size() - P->step_first[step];
+        // group the step's items by kernel variant (stable: tile order kept)
+        auto vid = [&](const WorkItem &w) { return variant_of(P->jobs[w.job]); };
+        std::stable_sort(P->items.begin() + P->step_first[step], P->items.end(),
+                         [&](const WorkItem &x, const WorkItem &y) { return vid(x) < vid(y); });
+        for (int i = P->step_first[step]; i < (int)P->items.size();) {
+            int j = i;
+            while (j < (int)P->items.size() && vid(P->items[j]) == vid(P->items[i])) ++j;
+            P->launches.push_back({i, j - i, vid(P->items[i])});
+            i = j;
+        }
     }
 
     const size_t jb = sizeof(LevelJob) * P->jobs.size();
@@ -759,11 +927,16 @@ static int plan_enqueue(BmiBatchPlan *P, hipStream_t s)
         hipLaunchKernelGGL(fused_persistent_kernel, dim3(groups), dim3(64 * wpg), 0, s, P->d_jobs,
                            P->d_items, n, P->d_counters, nj, groups * wpg);
     } else {
-        for (size_t st = 0; st < P->step_first.size(); ++st) {
-            const int cnt = P->step_count[st];
-            if (!cnt) continue;
-            hipLaunchKernelGGL(fused_step_kernel, dim3((cnt + wpg - 1) / wpg), dim3(64 * wpg), 0, s,
-                               P->d_jobs, P->d_items, P->step_first[st], cnt);
+        for (const auto &l : P->launches) {
+            const dim3 grid((l.count + wpg - 1) / wpg), block(64 * wpg);
+            switch (l.variant) {
+            case 0: hipLaunchKernelGGL((fused_step_kernel<53, 1>), grid, block, 0, s, P->d_jobs, P->d_items, l.first, l.count); break;
+            case 1: hipLaunchKernelGGL((fused_step_kernel<53, 3>), grid, block, 0, s, P->d_jobs, P->d_items, l.first, l.count); break;
+            case 2: hipLaunchKernelGGL((fused_step_kernel<53, 4>), grid, block, 0, s, P->d_jobs, P->d_items, l.first, l.count); break;
+            case 3: hipLaunchKernelGGL((fused_step_kernel<97, 1>), grid, block, 0, s, P->d_jobs, P->d_items, l.first, l.count); break;
+            case 4: hipLaunchKernelGGL((fused_step_kernel<97, 3>), grid, block, 0, s, P->d_jobs, P->d_items, l.first, l.count); break;
+            default: hipLaunchKernelGGL((fused_step_kernel<97, 4>), grid, block, 0, s, P->d_jobs, P->d_items, l.first, l.count); break;
+            }
         }
     }
     hipError_t e = hipGetLastError();
@@ -819,11 +992,12 @@ extern "C" int BMI_batch_plan_rebind(BmiBatchPlan *P, int i, const BmiTileDesc *
     for (size_t j = 0; j < P->jobs.size(); ++j) {
         if (P->job_tile[j] != i) continue;
         LevelJob &J = P->jobs[j];
-        for (int s = 0; s < J.nc; ++s) {
+        for (int s = 0; s < 4; ++s) {
+            if (s >= J.nc && J.nc == 1) break;
             const int c = J.comp[s];
             J.det[s] = t->coef[c];
             if (J.lev == 0) J.ll[s] = t->coef[c];
-            if (J.out_kind == OUT_PLANE) J.dst[s] = t->planes_out[c];
+            if (J.out_kind == OUT_PLANE && s < J.nc) J.dst[s] = t->planes_out[c];
         }
         if (J.out_kind == OUT_RGB || J.out_kind == OUT_MERGE) {
             J.out = t->out;

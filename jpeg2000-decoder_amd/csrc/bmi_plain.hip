@@ -52,11 +52,11 @@ __device__ inline typename PlainT<FILT>::T plain_load(const void *base, size_t i
                                                       float gain)
 {
     if constexpr (FILT == 97) {
-        const uint32_t w = ((const uint32_t *)base)[idx];
+        const uint32_t w = gptr<uint32_t>(base)[idx];
         const float v = (kind == SRC_SIGNMAG) ? (float)signmag_to_int(w) : __uint_as_float(w);
         return __fmul_rn(v, gain);
     } else {
-        return ((const int *)base)[idx];
+        return gptr<int>(base)[idx];
     }
 }
 
@@ -85,7 +85,11 @@ __global__ void __launch_bounds__(256) plain_line_kernel(LineArgs A)
     const int tid = threadIdx.x;
     const int n = A.n;
 
-    for (int e = tid; e < kPlLines * W; e += 256) {
+    constexpr int NLOAD = (kPlLines * W + 255) / 256;
+#pragma unroll
+    for (int it = 0; it < NLOAD; ++it) {
+        const int e = tid + 256 * it;
+        if (e >= kPlLines * W) break;
         const int l = e / W, p = e - l * W;
         const int line = line0 + l;
         T v = (T)0;
@@ -161,11 +165,13 @@ __global__ void __launch_bounds__(256) plain_line_kernel(LineArgs A)
         __syncthreads();
     }
 
-    for (int e = tid; e < kPlSeg * kPlLines; e += 256) {
+#pragma unroll
+    for (int it = 0; it < kPlSeg * kPlLines / 256; ++it) {
+        const int e = tid + 256 * it;
         const int p = e / kPlLines, l = e - p * kPlLines;
         const int i = seg0 + p, line = line0 + l;
         if (i < n && line < A.nlines)
-            ((T *)A.out)[(size_t)i * A.out_pitch + line] = x[l][H + p];
+            gptr_w<T>(A.out)[(size_t)i * A.out_pitch + line] = x[l][H + p];
     }
 }
 
