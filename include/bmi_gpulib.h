@@ -156,11 +156,13 @@ typedef struct BmiTileDesc {
 
 /* Launch shape of the fused kernel (0 = library default / autotuned). */
 typedef struct BmiTuning {
-    int seg_rows;        /* output rows per work item (even) */
+    int seg_rows;        /* maximum output rows per work item (even) */
     int waves_per_group; /* 1..16 wave64s per workgroup */
     int groups_per_cu;   /* persistent workgroups per CU */
     int persistent;      /* 1 = one persistent launch with a device work
-                            queue; 0 = one launch per level step */
+                            queue; 0 (default) = one launch per level step */
+    int min_waves_per_cu; /* segment heights shrink until each level step
+                             offers this many items per CU */
 } BmiTuning;
 
 typedef struct BmiBatchPlan BmiBatchPlan;
@@ -180,6 +182,12 @@ int BMI_batch_plan_execute_graph(BmiBatchPlan *plan, StreamId stream);
    frames. */
 int BMI_batch_plan_rebind(BmiBatchPlan *plan, int i, const BmiTileDesc *tile);
 size_t BMI_batch_plan_scratch_bytes(const BmiBatchPlan *plan);
+/* Level-step plans: number of kernel launches one execute() enqueues, and
+   enqueue only launches [first, first+count) -- for timing a single level
+   (launch order is the dependency order, so a range starting at 0 is a
+   valid partial decode). */
+int BMI_batch_plan_num_launches(const BmiBatchPlan *plan);
+int BMI_batch_plan_execute_launches(BmiBatchPlan *plan, int first, int count, StreamId stream);
 int BMI_batch_plan_num_items(const BmiBatchPlan *plan);
 int BMI_batch_plan_destroy(BmiBatchPlan *plan);
 

@@ -92,6 +92,7 @@ class BmiTuning(ctypes.Structure):
         ("waves_per_group", ctypes.c_int),
         ("groups_per_cu", ctypes.c_int),
         ("persistent", ctypes.c_int),
+        ("min_waves_per_cu", ctypes.c_int),
     ]
 
 
@@ -148,6 +149,10 @@ def lib() -> ctypes.CDLL:
     L.BMI_batch_plan_destroy.restype = i
     L.BMI_batch_plan_error_flag.argtypes = [vp]
     L.BMI_batch_plan_error_flag.restype = i
+    L.BMI_batch_plan_num_launches.argtypes = [vp]
+    L.BMI_batch_plan_num_launches.restype = i
+    L.BMI_batch_plan_execute_launches.argtypes = [vp, i, i, vp]
+    L.BMI_batch_plan_execute_launches.restype = i
     L.BMI_plain_decode.argtypes = [ctypes.POINTER(BmiTileDesc), vp, ctypes.c_size_t, vp]
     L.BMI_plain_decode.restype = i
     _ = f
@@ -212,6 +217,14 @@ class BatchPlan:
     def execute_graph(self, stream: int = 0) -> None:
         check(lib().BMI_batch_plan_execute_graph(self.handle, stream or None),
               "BMI_batch_plan_execute_graph")
+
+    @property
+    def num_launches(self) -> int:
+        return lib().BMI_batch_plan_num_launches(self.handle)
+
+    def execute_launches(self, first: int, count: int, stream: int = 0) -> None:
+        check(lib().BMI_batch_plan_execute_launches(self.handle, first, count, stream or None),
+              "BMI_batch_plan_execute_launches")
 
     def rebind(self, i: int, desc: BmiTileDesc) -> None:
         check(lib().BMI_batch_plan_rebind(self.handle, i, ctypes.byref(desc)), "rebind")

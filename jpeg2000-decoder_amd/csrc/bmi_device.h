@@ -75,12 +75,13 @@ __device__ inline float lift97(float x, float c, float l, float r)
     return __fsub_rn(x, p);
 }
 
-// wave64 neighbour exchange with DPP wave shifts (a VALU move, no LDS
-// round trip): wave_shr:1 gives each lane the value of lane-1, wave_shl:1
-// the value of lane+1.  Edge lanes keep their own value; they are halo
-// lanes and never stored.
-__device__ inline int wave_from_left(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
-__device__ inline int wave_from_right(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xf, 0xf, false); }
+// wave64 neighbour exchange with DPP wave shifts (a VALU modifier, no
+// LDS round trip): wave_shr:1 gives each lane the value of lane-1,
+// wave_shl:1 the value of lane+1.  With bound_ctrl the edge lanes read 0,
+// which lets the compiler fold the shift into the consuming add; edge
+// lanes are halo lanes and never stored.
+__device__ inline int wave_from_left(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
+__device__ inline int wave_from_right(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true); }
 __device__ inline float wave_from_left(float v) { return __int_as_float(wave_from_left(__float_as_int(v))); }
 __device__ inline float wave_from_right(float v) { return __int_as_float(wave_from_right(__float_as_int(v))); }
 
@@ -160,12 +161,11 @@ __host__ __device__ inline int fmt_bytes(int fmt)
 
 __device__ inline int clamp_i(int v, int mx) { return v < 0 ? 0 : (v > mx ? mx : v); }
 
-// (int)(SET_BOUNDARY(0, v, maxV)) with the comparison in float
+// (int)(SET_BOUNDARY(0, v, maxV)) with the comparison in float; med3 gives
+// the same value for every non-NaN v (including -0.0, which truncates to 0)
 __device__ inline int clamp_trunc_f(float v, int mx)
 {
-    const float fm = (float)mx;
-    const float c = v < 0.0f ? 0.0f : (v > fm ? fm : v);
-    return (int)c;
+    return (int)__builtin_amdgcn_fmed3f(v, 0.0f, (float)mx);
 }
 
 // inverse ICT + level shift + clamp + truncate + LIFT, float throughout

@@ -94,48 +94,198 @@ __device__ inline T half_sample(T v)
         return v >> 1;
 }
 
-// write one output position (all lockstep components of one pixel)
-template <int FILT, int NC>
-__device__ inline void emit_pixel(const LevelJob &J, int row, int col,
-                                  const typename SampleT<FILT>::T *v)
+// Output kinds of a work item (kernel template parameter)
+constexpr int K_PLANE = 0;  // float / int32 plane (intermediate or final)
+constexpr int K_RGB = 1;    // inverse RCT / ICT + pack
+constexpr int K_MERGE = 2;  // level shift + pack of 1..4 components
+
+// Pixel layouts (element order B,G,R[,A], codec_base.h:51-143)
+constexpr int PL_BGR8 = 0, PL_BGRA8 = 1, PL_BGR16 = 2, PL_BGRA16 = 3, PL_ANY = 4;
+
+__host__ __device__ inline int pixel_layout(int fmt)
 {
-    using T = typename SampleT<FILT>::T;
-    const int ok = J.out_kind;
-    if (ok == OUT_SCRATCH || ok == OUT_PLANE) {
-        size_t idx = (size_t)(row - J.b0) * J.dst_pitch + (col - J.a0);
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-            if (c < J.nc) gptr_w<T>(J.dst[c])[idx] = v[c];
-        return;
+    switch (fmt) {
+    case IMAGE_FOTMAT_24BITS_RGB:
+    case IMAGE_FOTMAT_24BITS_3COMPONENTS: return PL_BGR8;
+    case IMAGE_FOTMAT_32BITS_ARGB:
+    case IMAGE_FOTMAT_32BITS_4COMPONENTS: return PL_BGRA8;
+    case IMAGE_FOTMAT_48BITS_RGB:
+    case IMAGE_FOTMAT_48BITS_3COMPONENTS: return PL_BGR16;
+    default: return PL_BGRA16;
     }
-    const int bd = J.bit_depth;
-    const int od = fmt_depth(J.fmt);
-    int R = 0, G = 0, B = 0, A = 0;
-    if (ok == OUT_RGB) {
-        if constexpr (NC >= 3) {
-            if constexpr (FILT == 97)
-                ict_pixel(v[0], v[1], v[2], bd, od, J.check_out, R, G, B);
-            else
-                rct_pixel(v[0], v[1], v[2], bd, od, J.check_out, R, G, B);
-        }
-    } else {  // OUT_MERGE
-        int t[4] = {0, 0, 0, 0};
+}
+
+// Output parameters of one item, read from the descriptor once.
+template <int FILT, int NC, int OUTK, int PL>
+struct Emitter {
+    using T = typename SampleT<FILT>::T;
+    __attribute__((address_space(1))) T *dst[NC];
+    __attribute__((address_space(1))) uint8_t *out;
+    int pitch, a0, b0, ox, oy, bd, od, ls, rs, mx, check_out, fmt, nc;
+    float fmx, foff;
+    bool pair_fast;  // both pixels of a lane in one aligned store sequence
+
+    __device__ inline void init(const LevelJob &__restrict__ J)
+    {
+        a0 = J.a0;
+        b0 = J.b0;
+        nc = J.nc;
+        if constexpr (OUTK == K_PLANE) {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            if (c < J.nc) {
-                int s;
-                if constexpr (FILT == 97) s = (int)v[c];
-                else s = v[c];
-                t[c] = merge_value(s, bd, od);
+            for (int c = 0; c < NC; ++c) dst[c] = gptr_w<T>(J.dst[c]);
+            pitch = J.dst_pitch;
+            bool al = true;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) al = al && !((uintptr_t)J.dst[c] & 7);
+            pair_fast = al && !(a0 & 1) && !(pitch & 1);
+        } else {
+            out = gptr_w<uint8_t>(J.out);
+            pitch = J.out_pitch_px;
+            ox = J.out_x - J.a0;
+            oy = J.out_y - J.b0;
+            bd = J.bit_depth;
+            fmt = J.fmt;
+            od = fmt_depth(fmt);
+            ls = od > bd ? od - bd : 0;
+            rs = bd > od ? bd - od : 0;
+            mx = (1 << bd) - 1;
+            fmx = (float)mx;
+            foff = (float)(1 << (bd - 1));
+            check_out = J.check_out;
+            pair_fast = (PL == PL_BGR8) && !((uintptr_t)J.out & 3) && !((pitch * 3) & 3) && !(ox & 1);
+        }
+    }
+
+    __device__ inline int lift(int v) const { return (v << ls) >> rs; }
+
+    // colour of one pixel (before packing)
+    __device__ inline void colour(const T *v, int &R, int &G, int &B, int &A) const
+    {
+        A = 0;
+        if constexpr (OUTK == K_RGB) {
+            if constexpr (NC >= 3) {
+                if constexpr (FILT == 97) {
+                    // RICT_97_FLOAT, bmi_cpu_dwt_decoder.cpp:3570-3841
+                    const float r = __fadd_rn(__fadd_rn(v[0], __fmul_rn(kCrR, v[2])), foff);
+                    const float g = __fadd_rn(
+                        __fadd_rn(__fadd_rn(v[0], __fmul_rn(kCbG, v[1])), __fmul_rn(kCrG, v[2])), foff);
+                    const float b = __fadd_rn(__fadd_rn(v[0], __fmul_rn(kCbB, v[1])), foff);
+                    R = lift((int)__builtin_amdgcn_fmed3f(r, 0.0f, fmx));
+                    G = lift((int)__builtin_amdgcn_fmed3f(g, 0.0f, fmx));
+                    B = lift((int)__builtin_amdgcn_fmed3f(b, 0.0f, fmx));
+                } else {
+                    // RRct_53, bmi_cpu_dwt_decoder.cpp:3936-4130
+                    const int g = v[0] - ((v[1] + v[2]) >> 2) + (1 << (bd - 1));
+                    R = lift(min(max(v[2] + g, 0), mx));
+                    G = lift(min(max(g, 0), mx));
+                    B = lift(min(max(v[1] + g, 0), mx));
+                }
+                if (check_out == 0) G = B = R;
+                else if (check_out == 1) R = B = G;
+                else if (check_out == 2) R = G = B;
+            } else {
+                R = G = B = 0;
+            }
+        } else {
+            int t[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                if (c < nc) {
+                    int sv;
+                    if constexpr (FILT == 97) sv = (int)v[c];
+                    else sv = v[c];
+                    t[c] = lift(min(max(sv + (1 << (bd - 1)), 0), mx));
+                }
+            }
+            merge_slots(t, fmt, check_out == 0, R, G, B, A);
+        }
+    }
+
+    __device__ inline void store(__attribute__((address_space(1))) uint8_t *p, int R, int G, int B,
+                                 int A) const
+    {
+        const int lay = (PL == PL_ANY) ? pixel_layout(fmt) : PL;
+        if (lay == PL_BGR8) {
+            p[0] = (uint8_t)B;
+            p[1] = (uint8_t)G;
+            p[2] = (uint8_t)R;
+        } else if (lay == PL_BGRA8) {
+            *(__attribute__((address_space(1))) uint32_t *)p =
+                (uint32_t)(B & 0xff) | ((uint32_t)(G & 0xff) << 8) | ((uint32_t)(R & 0xff) << 16) |
+                ((uint32_t)(A & 0xff) << 24);
+        } else if (lay == PL_BGR16) {
+            auto q = (__attribute__((address_space(1))) uint16_t *)p;
+            q[0] = (uint16_t)B;
+            q[1] = (uint16_t)G;
+            q[2] = (uint16_t)R;
+        } else {
+            auto q = (__attribute__((address_space(1))) uint16_t *)p;
+            q[0] = (uint16_t)B;
+            q[1] = (uint16_t)G;
+            q[2] = (uint16_t)R;
+            q[3] = (uint16_t)A;
+        }
+    }
+
+    __device__ inline int layout_bytes() const
+    {
+        const int lay = (PL == PL_ANY) ? pixel_layout(fmt) : PL;
+        return lay == PL_BGR8 ? 3 : lay == PL_BGRA8 ? 4 : lay == PL_BGR16 ? 6 : 8;
+    }
+
+    __device__ inline void put(int row, int col, const T *v) const
+    {
+        if constexpr (OUTK == K_PLANE) {
+            const size_t idx = (size_t)(row - b0) * pitch + (col - a0);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) dst[c][idx] = v[c];
+        } else {
+            int R, G, B, A;
+            colour(v, R, G, B, A);
+            store(out + ((size_t)(oy + row) * pitch + (ox + col)) * layout_bytes(), R, G, B, A);
+        }
+    }
+
+    // the two pixels of a lane, (row, ce) and (row, ce + 1)
+    __device__ inline void put2(int row, int ce, const T *ve, const T *vo, bool e_ok, bool o_ok) const
+    {
+        if (pair_fast && e_ok && o_ok) {
+            if constexpr (OUTK == K_PLANE) {
+                const size_t idx = (size_t)(row - b0) * pitch + (ce - a0);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    typedef T v2 __attribute__((ext_vector_type(2)));
+                    v2 pairv;
+                    pairv.x = ve[c];
+                    pairv.y = vo[c];
+                    *(__attribute__((address_space(1))) v2 *)(dst[c] + idx) = pairv;
+                }
+                return;
+            } else if constexpr (PL == PL_BGR8) {
+                // 6 bytes B0 G0 R0 B1 G1 R1 at an even, 2-aligned offset:
+                // one aligned dword and one aligned short
+                int R0, G0, B0, A0, R1, G1, B1, A1;
+                colour(ve, R0, G0, B0, A0);
+                colour(vo, R1, G1, B1, A1);
+                const size_t off = ((size_t)(oy + row) * pitch + (ox + ce)) * 3;
+                const bool mid = (off & 2) != 0;
+                const uint32_t lo4 = (uint32_t)(B0 & 0xff) | ((uint32_t)(G0 & 0xff) << 8) |
+                                     ((uint32_t)(R0 & 0xff) << 16) | ((uint32_t)(B1 & 0xff) << 24);
+                const uint32_t hi4 = (uint32_t)(R0 & 0xff) | ((uint32_t)(B1 & 0xff) << 8) |
+                                     ((uint32_t)(G1 & 0xff) << 16) | ((uint32_t)(R1 & 0xff) << 24);
+                const uint32_t sh_lo = (uint32_t)(B0 & 0xff) | ((uint32_t)(G0 & 0xff) << 8);
+                const uint32_t sh_hi = (uint32_t)(G1 & 0xff) | ((uint32_t)(R1 & 0xff) << 8);
+                const size_t dw_off = mid ? off + 2 : off;
+                const size_t sh_off = mid ? off : off + 4;
+                *(__attribute__((address_space(1))) uint32_t *)(out + dw_off) = mid ? hi4 : lo4;
+                *(__attribute__((address_space(1))) uint16_t *)(out + sh_off) = (uint16_t)(mid ? sh_lo : sh_hi);
+                return;
             }
         }
-        merge_slots(t, J.fmt, J.check_out == 0, R, G, B, A);
+        if (e_ok) put(row, ce, ve);
+        if (o_ok) put(row, ce + 1, vo);
     }
-    uint8_t *p = (uint8_t *)J.out +
-                 ((size_t)(J.out_y + row - J.b0) * J.out_pitch_px + (J.out_x + col - J.a0)) *
-                     fmt_bytes(J.fmt);
-    put_pixel(p, J.fmt, R, G, B, A);
-}
+};
 
 // Prefetch depth of the row-pair pipeline: loads for pair k+D are issued
 // while pair k is lifted, so each wave keeps D pairs of loads in flight
@@ -145,13 +295,23 @@ struct Prefetch {
     static constexpr int D = (NC == 1) ? 4 : 2;
 };
 
+// Coefficient encodings of a 9/7 item, fixed per kernel variant: LL and
+// detail bands float (KD_FF), LL float and details sign-magnitude (KD_FS,
+// levels above 0 of T1 data), both sign-magnitude (KD_SS), or chosen at
+// run time (KD_ANY)
+constexpr int KD_ANY = 0, KD_FF = 1, KD_FS = 2, KD_SS = 3;
+constexpr int SK_RUNTIME = 0, SK_FLOAT = 1, SK_SIGNMAG = 2;
+
 // convert a raw 32-bit word to a sample: sign-magnitude or float words
-// times the folded gain (9/7), two's complement (5/3); select, no branch
-template <int FILT>
+// times the folded gain (9/7), two's complement (5/3)
+template <int FILT, int SK>
 __device__ inline typename SampleT<FILT>::T to_sample(uint32_t w, int is_float, float gain)
 {
     if constexpr (FILT == 97) {
-        const float v = is_float ? __uint_as_float(w) : (float)signmag_to_int(w);
+        float v;
+        if constexpr (SK == SK_FLOAT) v = __uint_as_float(w);
+        else if constexpr (SK == SK_SIGNMAG) v = (float)signmag_to_int(w);
+        else v = is_float ? __uint_as_float(w) : (float)signmag_to_int(w);
         return __fmul_rn(v, gain);
     } else {
         return (int)w;
@@ -159,16 +319,18 @@ __device__ inline typename SampleT<FILT>::T to_sample(uint32_t w, int is_float, 
 }
 
 // Degenerate levels (a single row or a single column): no prefetching.
-template <int FILT, int NC>
+template <int FILT, int NC, int OUTK, int PL>
 __device__ __attribute__((always_inline)) inline void process_item_small(const LevelJob &__restrict__ J,
                                                              int strip, int seg, int lane);
 
 // One work item: strip `strip`, row segment `seg` of level job J, run by
 // one wave64 (lane = 0..63).
-template <int FILT, int NC>
+template <int FILT, int NC, int OUTK, int PL, int KD>
 __device__ __attribute__((always_inline)) inline void process_item(
     const LevelJob &__restrict__ J, int strip, int seg, int lane)
 {
+    constexpr int SK_LL = (KD == KD_FF || KD == KD_FS) ? SK_FLOAT : (KD == KD_SS ? SK_SIGNMAG : SK_RUNTIME);
+    constexpr int SK_DET = (KD == KD_FF) ? SK_FLOAT : ((KD == KD_FS || KD == KD_SS) ? SK_SIGNMAG : SK_RUNTIME);
     using T = typename SampleT<FILT>::T;
     constexpr int HP = (FILT == 97) ? 2 : 1;  // halo pairs on each side
     constexpr int OWP = 64 - 2 * HP;          // output pairs per strip
@@ -176,7 +338,7 @@ __device__ __attribute__((always_inline)) inline void process_item(
 
     const int a0 = J.a0, a1 = J.a1, b0 = J.b0, b1 = J.b1;
     if ((a1 - a0) == 1 || (b1 - b0) == 1) {
-        process_item_small<FILT, NC>(J, strip, seg, lane);
+        process_item_small<FILT, NC, OUTK, PL>(J, strip, seg, lane);
         return;
     }
     // every field the loop needs, hoisted out of the descriptor once
@@ -209,8 +371,15 @@ __device__ __attribute__((always_inline)) inline void process_item(
     const int kout1 = min(Klast, kout0 + seg_pairs - 1);
 
     // raw words of pair k: [comp][top-low, top-high, bottom-low, bottom-high]
+    // levels of 8+ rows need at most one reflection for the rows a segment
+    // touches (2 row pairs of halo beyond each end)
+    const bool tall = (b1 - b0) >= 8;
+    auto reflect_row = [&](int r) {
+        if (tall) return r < b0 ? 2 * b0 - r : (r >= b1 ? 2 * (b1 - 1) - r : r);
+        return reflect(r, b0, b1);
+    };
     auto fetch = [&](int k, uint32_t (&r)[NC][4]) {
-        const int rre = reflect(2 * k, b0, b1), rro = reflect(2 * k + 1, b0, b1);
+        const int rre = reflect_row(2 * k), rro = reflect_row(2 * k + 1);
         const size_t rt = (size_t)((rre >> 1) - rl0);
         const size_t q = (size_t)((rro >> 1) - rh0 + lh);
 #pragma unroll
@@ -222,10 +391,13 @@ __device__ __attribute__((always_inline)) inline void process_item(
         }
     };
 
+    Emitter<FILT, NC, OUTK, PL> em;
+    em.init(J);
+    const bool ce_ok = lane_out && ce >= a0 && ce < a1;
+    const bool co_ok = lane_out && co >= a0 && co < a1;
     auto emit_row = [&](int row, const T *ve, const T *vo) {
-        if (!lane_out || row < b0 || row >= b1) return;
-        if (ce >= a0 && ce < a1) emit_pixel<FILT, NC>(J, row, ce, ve);
-        if (co >= a0 && co < a1) emit_pixel<FILT, NC>(J, row, co, vo);
+        if (row < b0 || row >= b1) return;
+        em.put2(row, ce, ve, vo, ce_ok, co_ok);
     };
 
     // vertical pipeline state per component and column (0 = ce, 1 = co)
@@ -246,15 +418,12 @@ __device__ __attribute__((always_inline)) inline void process_item(
     const int ks = kout0 - ((FILT == 97) ? 2 : (quirk ? 2 : 1));
     const int ke = kout1 + LAG;
 
-    // one pipeline step with pair k
-    auto consume = [&](int k, const uint32_t (&raw)[NC][4]) {
+    // one pipeline step with pair k (samples already converted)
+    auto consume = [&](int k, T (&smp)[NC][4]) {
         T evn[2][NC], odd[2][NC];  // [column][component]
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-            T tl = to_sample<FILT>(raw[c][0], ll_float, gn[c][0]);
-            T th = to_sample<FILT>(raw[c][1], det_float, gn[c][1]);
-            T bl = to_sample<FILT>(raw[c][2], det_float, gn[c][2]);
-            T bh = to_sample<FILT>(raw[c][3], det_float, gn[c][3]);
+            T tl = smp[c][0], th = smp[c][1], bl = smp[c][2], bh = smp[c][3];
             hlift<FILT>(tl, th);
             hlift<FILT>(bl, bh);
             const T Lk[2] = {tl, th}, Hk[2] = {bl, bh};
@@ -311,18 +480,24 @@ __device__ __attribute__((always_inline)) inline void process_item(
 #pragma unroll
         for (int d = 0; d < D; ++d) {
             const int k = ks + it * D + d;
-            uint32_t cur[NC][4];
+            // convert slot d (its loads were issued D pairs ago), then reuse
+            // its registers for pair k + D before doing the arithmetic
+            T smp[NC][4];
 #pragma unroll
-            for (int c = 0; c < NC; ++c)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) cur[c][j] = ring[d][c][j];
+            for (int c = 0; c < NC; ++c) {
+                smp[c][0] = to_sample<FILT, SK_LL>(ring[d][c][0], ll_float, gn[c][0]);
+                smp[c][1] = to_sample<FILT, SK_DET>(ring[d][c][1], det_float, gn[c][1]);
+                smp[c][2] = to_sample<FILT, SK_DET>(ring[d][c][2], det_float, gn[c][2]);
+                smp[c][3] = to_sample<FILT, SK_DET>(ring[d][c][3], det_float, gn[c][3]);
+            }
             fetch(min(k + D, ke), ring[d]);
-            consume(k, cur);
+            consume(k, smp);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
 
-template <int FILT, int NC>
+template <int FILT, int NC, int OUTK, int PL>
 __device__ __attribute__((always_inline)) inline void process_item_small(const LevelJob &__restrict__ J,
                                                              int strip, int seg, int lane)
 {
@@ -381,10 +556,13 @@ __device__ __attribute__((always_inline)) inline void process_item_small(const L
             bl = bh = half_sample<FILT>(bh);
         }
     };
+    Emitter<FILT, NC, OUTK, PL> em;
+    em.init(J);
+    const bool ce_ok = lane_out && ce >= a0 && ce < a1;
+    const bool co_ok = lane_out && co >= a0 && co < a1;
     auto emit_row = [&](int row, const T *ve, const T *vo) {
-        if (!lane_out || row < b0 || row >= b1) return;
-        if (ce >= a0 && ce < a1) emit_pixel<FILT, NC>(J, row, ce, ve);
-        if (co >= a0 && co < a1) emit_pixel<FILT, NC>(J, row, co, vo);
+        if (row < b0 || row >= b1) return;
+        em.put2(row, ce, ve, vo, ce_ok, co_ok);
     };
 
     if (single_row) {
@@ -464,26 +642,60 @@ __device__ __attribute__((always_inline)) inline void process_item_small(const L
     }
 }
 
-template <int FILT>
-__device__ __attribute__((always_inline)) inline void dispatch_nc(const LevelJob &__restrict__ J, int strip, int seg, int lane)
+// Kernel variants: V = (filt == 97) * 32 + sub * 4 + kd.
+// sub: 0 plane (1 comp), 1 merge (1 comp), 2..5 RGB of layout
+// PL_BGR8..PL_BGRA16 (3 comps), 6 merge (2-3 comps), 7 merge (4 comps).
+// kd: coefficient encodings (9/7 planes and RGB only; KD_ANY otherwise).
+__host__ __device__ inline int variant_of(const LevelJob &J)
 {
-    if (J.nc == 1)
-        process_item<FILT, 1>(J, strip, seg, lane);
-    else if (J.nc <= 3)
-        process_item<FILT, 3>(J, strip, seg, lane);
-    else
-        process_item<FILT, 4>(J, strip, seg, lane);
+    int sub;
+    if (J.out_kind == OUT_SCRATCH || J.out_kind == OUT_PLANE) sub = 0;
+    else if (J.out_kind == OUT_RGB) sub = 2 + pixel_layout(J.fmt);
+    else sub = J.nc == 1 ? 1 : (J.nc <= 3 ? 6 : 7);
+    int kd = KD_ANY;
+    if (J.filt == 97 && (sub == 0 || (sub >= 2 && sub <= 5))) {
+        const bool llf = J.ll_kind == SRC_FLOAT, detf = J.det_kind == SRC_FLOAT;
+        kd = (llf && detf) ? KD_FF : (llf ? KD_FS : (detf ? KD_ANY : KD_SS));
+    }
+    return (J.filt == 97 ? 32 : 0) + sub * 4 + kd;
 }
 
-__device__ __attribute__((always_inline)) inline void run_item(const LevelJob *__restrict__ jobs, const WorkItem &it, int lane)
+template <int V>
+__device__ __attribute__((always_inline)) inline void run_variant(const LevelJob &__restrict__ J,
+                                                                 int strip, int seg, int lane)
+{
+    constexpr int FILT = (V >= 32) ? 97 : 53;
+    constexpr int SUB = (V >> 2) & 7;
+    constexpr int KD = V & 3;
+    if constexpr (SUB == 0) process_item<FILT, 1, K_PLANE, PL_ANY, KD>(J, strip, seg, lane);
+    else if constexpr (SUB == 1) process_item<FILT, 1, K_MERGE, PL_ANY, KD>(J, strip, seg, lane);
+    else if constexpr (SUB >= 2 && SUB <= 5) process_item<FILT, 3, K_RGB, SUB - 2, KD>(J, strip, seg, lane);
+    else if constexpr (SUB == 6) process_item<FILT, 3, K_MERGE, PL_ANY, KD>(J, strip, seg, lane);
+    else process_item<FILT, 4, K_MERGE, PL_ANY, KD>(J, strip, seg, lane);
+}
+
+// every instantiated variant, as X(V)
+#define BMI_FOR_EACH_VARIANT(X)                                                             \
+    X(0) X(4) X(8) X(12) X(16) X(20) X(24) X(28)                                            \
+    X(32) X(33) X(34) X(35) X(36) X(40) X(41) X(42) X(43) X(44) X(45) X(46) X(47) X(48)     \
+    X(49) X(50) X(51) X(52) X(53) X(54) X(55) X(56) X(60)
+
+__device__ __attribute__((always_inline)) inline void run_item(
+    const LevelJob *__restrict__ jobs, const WorkItem &it, int lane)
 {
     const LevelJob &__restrict__ J = jobs[it.job];
+    // consecutive items are neighbouring strips of one row segment: waves
+    // running together read whole image rows, sharing the cache lines that
+    // straddle strip boundaries
     const int strip = it.local % J.nstrips;
     const int seg = it.local / J.nstrips;
-    if (J.filt == 97)
-        dispatch_nc<97>(J, strip, seg, lane);
-    else
-        dispatch_nc<53>(J, strip, seg, lane);
+    switch (variant_of(J)) {
+#define BMI_RUN_CASE(V) \
+    case V: run_variant<V>(J, strip, seg, lane); break;
+        BMI_FOR_EACH_VARIANT(BMI_RUN_CASE)
+#undef BMI_RUN_CASE
+    default: break;
+    }
 }
 
 // spin limit: a dependency that never completes (a bug) ends the wait and
@@ -551,7 +763,7 @@ __global__ void __launch_bounds__(256) fused_persistent_kernel(const LevelJob *j
 
 // level-step mode: one launch per dependency step and item variant (no
 // waiting); each variant gets its own register budget
-template <int FILT, int NC>
+template <int V>
 __global__ void __launch_bounds__(256) fused_step_kernel(const LevelJob *__restrict__ jobs,
                                                          const WorkItem *__restrict__ items,
                                                          int first, int count)
@@ -563,7 +775,7 @@ __global__ void __launch_bounds__(256) fused_step_kernel(const LevelJob *__restr
     const int job = __builtin_amdgcn_readfirstlane(items[first + i].job);
     const int local = __builtin_amdgcn_readfirstlane(items[first + i].local);
     const LevelJob &__restrict__ J = jobs[job];
-    process_item<FILT, NC>(J, local % J.nstrips, local / J.nstrips, lane);
+    run_variant<V>(J, local % J.nstrips, local / J.nstrips, lane);
 }
 
 }  // namespace bmi
@@ -584,7 +796,9 @@ struct BmiBatchPlan {
     };
     std::vector<Launch> launches;  // level-step mode: per step and variant
     std::vector<int> job_tile;                // tile of each job
-    BmiTuning tuning;
+    struct {
+        int seg_rows, waves_per_group, groups_per_cu, persistent, min_waves_per_cu;
+    } tuning;
     LevelJob *d_jobs = nullptr;
     WorkItem *d_items = nullptr;
     int *d_counters = nullptr;  // [njobs] completion counters, [njobs] = head,
@@ -601,13 +815,6 @@ struct BmiBatchPlan {
 };
 
 namespace {
-
-// kernel variant of a job: filter x lockstep width class
-int variant_of(const LevelJob &J)
-{
-    const int ncc = J.nc == 1 ? 0 : (J.nc <= 3 ? 1 : 2);
-    return (J.filt == 97 ? 3 : 0) + ncc;
-}
 
 int out_kind_for(const BmiTileDesc &t)
 {
@@ -697,7 +904,8 @@ extern "C" int BMI_batch_plan_create(BmiBatchPlan **out_plan, const BmiTileDesc 
     if (P->tuning.seg_rows & 1) P->tuning.seg_rows += 1;
     P->tuning.waves_per_group = (tuning && tuning->waves_per_group > 0) ? tuning->waves_per_group : 4;
     P->tuning.groups_per_cu = (tuning && tuning->groups_per_cu > 0) ? tuning->groups_per_cu : 8;
-    P->tuning.persistent = tuning ? tuning->persistent : 1;
+    P->tuning.persistent = tuning ? tuning->persistent : 0;
+    P->tuning.min_waves_per_cu = (tuning && tuning->min_waves_per_cu > 0) ? tuning->min_waves_per_cu : 8;
     if (P->tuning.waves_per_group > 4) P->tuning.waves_per_group = 4;
     if (hipGetDevice(&P->device) != hipSuccess) P->device = 0;
     {
@@ -753,6 +961,7 @@ extern "C" int BMI_batch_plan_create(BmiBatchPlan **out_plan, const BmiTileDesc 
     P->step_count.assign(max_steps, 0);
     for (int step = 0; step < max_steps; ++step) {
         P->step_first[step] = (int)P->items.size();
+        const int step_job0 = (int)P->jobs.size();
         for (int ti = 0; ti < ntiles; ++ti) {
             const BmiTileDesc &t = tiles[ti];
             const int L = t.levels;
@@ -832,7 +1041,6 @@ extern "C" int BMI_batch_plan_create(BmiBatchPlan **out_plan, const BmiTileDesc 
                 P->jobs.push_back(J);
                 P->job_tile.push_back(ti);
                 if (key_comp >= 0) job_of[ti][lev * 4 + key_comp] = jid;
-                for (int li = 0; li < J.n_items; ++li) P->items.push_back(WorkItem{jid, li});
                 return jid;
             };
 
@@ -883,6 +1091,31 @@ extern "C" int BMI_batch_plan_create(BmiBatchPlan **out_plan, const BmiTileDesc 
                 push_job(J, -1);
             }
         }
+        // segment height: start at the tuned maximum and halve it for all
+        // jobs of the step until the step has enough items to fill the GPU
+        // (small coarse levels are latency-bound, not bandwidth-bound)
+        {
+            const int target = P->num_cus * P->tuning.min_waves_per_cu;
+            for (;;) {
+                long total = 0;
+                bool can = false;
+                for (int j = step_job0; j < (int)P->jobs.size(); ++j) {
+                    total += P->jobs[j].n_items;
+                    can = can || P->jobs[j].seg_pairs > 1;
+                }
+                if (total >= target || !can) break;
+                for (int j = step_job0; j < (int)P->jobs.size(); ++j) {
+                    LevelJob &J = P->jobs[j];
+                    if (J.seg_pairs <= 1) continue;
+                    J.seg_pairs >>= 1;
+                    const int nrows = J.b1 - (J.b0 & ~1);
+                    J.nsegs = (nrows + 2 * J.seg_pairs - 1) / (2 * J.seg_pairs);
+                    J.n_items = J.nstrips * J.nsegs;
+                }
+            }
+            for (int j = step_job0; j < (int)P->jobs.size(); ++j)
+                for (int li = 0; li < P->jobs[j].n_items; ++li) P->items.push_back(WorkItem{j, li});
+        }
         P->step_count[step] = (int)P->items.size() - P->step_first[step];
         // group the step's items by kernel variant (stable: tile order kept)
         auto vid = [&](const WorkItem &w) { return variant_of(P->jobs[w.job]); };
@@ -930,18 +1163,40 @@ static int plan_enqueue(BmiBatchPlan *P, hipStream_t s)
         for (const auto &l : P->launches) {
             const dim3 grid((l.count + wpg - 1) / wpg), block(64 * wpg);
             switch (l.variant) {
-            case 0: hipLaunchKernelGGL((fused_step_kernel<53, 1>), grid, block, 0, s, P->d_jobs, P->d_items, l.first, l.count); break;
-            case 1: hipLaunchKernelGGL((fused_step_kernel<53, 3>), grid, block, 0, s, P->d_jobs, P->d_items, l.first, l.count); break;
-            case 2: hipLaunchKernelGGL((fused_step_kernel<53, 4>), grid, block, 0, s, P->d_jobs, P->d_items, l.first, l.count); break;
-            case 3: hipLaunchKernelGGL((fused_step_kernel<97, 1>), grid, block, 0, s, P->d_jobs, P->d_items, l.first, l.count); break;
-            case 4: hipLaunchKernelGGL((fused_step_kernel<97, 3>), grid, block, 0, s, P->d_jobs, P->d_items, l.first, l.count); break;
-            default: hipLaunchKernelGGL((fused_step_kernel<97, 4>), grid, block, 0, s, P->d_jobs, P->d_items, l.first, l.count); break;
+#define BMI_STEP_CASE(V)                                                                        \
+    case V:                                                                                    \
+        hipLaunchKernelGGL((fused_step_kernel<V>), grid, block, 0, s, P->d_jobs, P->d_items, \
+                           l.first, l.count);                                                  \
+        break;
+                BMI_FOR_EACH_VARIANT(BMI_STEP_CASE)
+#undef BMI_STEP_CASE
+            default:
+                break;
             }
         }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return bmi_fail(BMI_E_HIP, "fused launch: %s", hipGetErrorString(e));
     return BMI_OK;
+}
+
+extern "C" int BMI_batch_plan_num_launches(const BmiBatchPlan *P)
+{
+    if (!P) return 0;
+    return P->tuning.persistent ? 1 : (int)P->launches.size();
+}
+
+extern "C" int BMI_batch_plan_execute_launches(BmiBatchPlan *P, int first, int count, StreamId stream)
+{
+    if (!P || P->tuning.persistent || first < 0 || count < 0 ||
+        first + count > (int)P->launches.size())
+        return bmi_fail(BMI_E_INVALID, "batch_plan_execute_launches: bad range");
+    auto saved = P->launches;
+    std::vector<BmiBatchPlan::Launch> sub(saved.begin() + first, saved.begin() + first + count);
+    P->launches.swap(sub);
+    int rc = plan_enqueue(P, (hipStream_t)stream);
+    P->launches.swap(sub);
+    return rc;
 }
 
 extern "C" int BMI_batch_plan_execute(BmiBatchPlan *P, StreamId stream)

@@ -111,25 +111,43 @@ __global__ void __launch_bounds__(256) plain_line_kernel(LineArgs A)
     __syncthreads();
 
     if (n > 1) {
+        // one update class per step (low samples on even steps); a step reads
+        // only the other class, so all of a thread's reads can be in flight
+        // together: fixed trip counts, fully unrolled
         constexpr int NSTEPS = (FILT == 97) ? 4 : 2;
+        constexpr int NE = kPlLines * W;
+        constexpr int NIT = (NE + 255) / 256;
 #pragma unroll
-        for (int s = 0; s < NSTEPS; ++s) {
-            const int want_low = (s & 1) == 0;  // even steps update low samples
-            for (int e = tid; e < kPlLines * W; e += 256) {
+        for (int st = 0; st < NSTEPS; ++st) {
+            const int want_low = (st & 1) == 0;
+            T nv[NIT];
+#pragma unroll
+            for (int it = 0; it < NIT; ++it) {
+                const int e = tid + 256 * it;
                 const int l = e / W, p = e - l * W;
-                if (p == 0 || p == W - 1) continue;
                 const int i = seg0 - H + p;
                 const bool low = (((i + A.parity) & 1) == 0);
-                if (low != (bool)want_low) continue;
-                if constexpr (FILT == 97) {
-                    const float c = s == 0 ? kDelta : s == 1 ? kGamma : s == 2 ? kBeta : kAlpha;
-                    x[l][p] = lift97(x[l][p], c, x[l][p - 1], x[l][p + 1]);
-                } else {
-                    if (s == 0)
-                        x[l][p] = x[l][p] - ((x[l][p - 1] + x[l][p + 1] + 2) >> 2);
-                    else
-                        x[l][p] = x[l][p] + ((x[l][p - 1] + x[l][p + 1]) >> 1);
+                const bool upd = e < NE && p > 0 && p < W - 1 && low == (bool)want_low;
+                nv[it] = (T)0;
+                if (upd) {
+                    if constexpr (FILT == 97) {
+                        const float c = st == 0 ? kDelta : st == 1 ? kGamma : st == 2 ? kBeta : kAlpha;
+                        nv[it] = lift97(x[l][p], c, x[l][p - 1], x[l][p + 1]);
+                    } else {
+                        if (st == 0)
+                            nv[it] = x[l][p] - ((x[l][p - 1] + x[l][p + 1] + 2) >> 2);
+                        else
+                            nv[it] = x[l][p] + ((x[l][p - 1] + x[l][p + 1]) >> 1);
+                    }
                 }
+            }
+#pragma unroll
+            for (int it = 0; it < NIT; ++it) {
+                const int e = tid + 256 * it;
+                const int l = e / W, p = e - l * W;
+                const int i = seg0 - H + p;
+                const bool low = (((i + A.parity) & 1) == 0);
+                if (e < NE && p > 0 && p < W - 1 && low == (bool)want_low) x[l][p] = nv[it];
             }
             __syncthreads();
         }

@@ -119,6 +119,6 @@ def test_fused_launch_shapes_agree(pkg, workloads, orc, seg_rows, persistent):
     w, h, L = 301, 199, 5
     planes = workloads.tile_planes(w, h, L, "97", 3, seed=8, ll97=(0.0, 40.0))
     _, exp = orc.decode_tile(planes, levels=L, filt="97", out_format=2)
-    t = pkg.BmiTuning(seg_rows, 4, 4, persistent)
+    t = pkg.BmiTuning(seg_rows, 4, 4, persistent, 0)
     _, got = run_fused(pkg, planes, tuning=t, filt="97", levels=L, out_format=2)
     assert np.array_equal(got, exp), diff_report(got, exp)
