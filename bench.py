@@ -167,8 +167,18 @@ def main() -> int:
     # and writes every 24-bit output pixel once
     k_bytes = W * H * 3 * 4 + W * H * 3
     k_gbs = k_bytes / (k_avg * 1e-3) / 1e9
+    # HBM traffic of that kernel per launch, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes (scaled by a calibration copy)
+    traffic = None
+    prof = ROOT / "profiles" / "fused_idwt97_ict_4096x2160x3.json"
+    if prof.exists() and (W, H) == (4096, 2160):
+        pj = json.loads(prof.read_text())
+        for kk in pj["kernels"]:
+            if "fused_step_kernel<41>" in kk["kernel"]:
+                traffic = int(kk["read_bytes"] + kk["write_bytes"])
     roofline = {"bound": "hbm", "achieved": round(k_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": "profiles/fused_idwt97_ict_4096x2160x3.json" if traffic else None,
                 "kernel": "fused_step_kernel<41> (final level: 9/7 + ICT + 24-bit pack)",
                 "kernel_us": round(k_avg * 1e3, 2), "algorithmic_bytes_per_launch": k_bytes}
     # whole decode: compulsory bytes of the job (coefficients in, pixels out)
