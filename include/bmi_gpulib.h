@@ -179,7 +179,7 @@ int BMI_batch_plan_execute(BmiBatchPlan *plan, StreamId stream);
 int BMI_batch_plan_execute_graph(BmiBatchPlan *plan, StreamId stream);
 /* Replace device pointers (coefficients / outputs) of tile i keeping its
    geometry, so one plan (and its graph) serves a stream of same-shaped
-   frames. */
+   frames.  Synchronous upload: no execute of this plan may be in flight. */
 int BMI_batch_plan_rebind(BmiBatchPlan *plan, int i, const BmiTileDesc *tile);
 size_t BMI_batch_plan_scratch_bytes(const BmiBatchPlan *plan);
 /* Level-step plans: number of kernel launches one execute() enqueues, and
@@ -190,6 +190,18 @@ int BMI_batch_plan_num_launches(const BmiBatchPlan *plan);
 int BMI_batch_plan_execute_launches(BmiBatchPlan *plan, int first, int count, StreamId stream);
 int BMI_batch_plan_num_items(const BmiBatchPlan *plan);
 int BMI_batch_plan_destroy(BmiBatchPlan *plan);
+/* Non-zero when a persistent launch gave up waiting for a dependency
+   (bounded spin) instead of hanging; synchronous. */
+int BMI_batch_plan_error_flag(BmiBatchPlan *plan);
+
+/* Autotuner: build a plan for `tiles` with each candidate tuning, run it
+   once untimed and `reps` times timed with HIP events on `stream`, and
+   return the fastest in *best (its mean time per execute in *best_us).
+   candidates == NULL selects the built-in grid (segment rows 16..128 x
+   items per CU 4..16 x 2 or 4 waves per workgroup, step mode).  Runs the
+   real decode: outputs of the tiles are written. */
+int BMI_autotune(const BmiTileDesc *tiles, int ntiles, const BmiTuning *candidates, int ncand,
+                 int reps, StreamId stream, BmiTuning *best, float *best_us);
 
 /* Plain path: the reference's own sequence of drop-in calls for one tile
    (BMI_comp_idwt per component, then BMI_tile_ICT / BMI_tile_MCT /
@@ -197,6 +209,68 @@ int BMI_batch_plan_destroy(BmiBatchPlan *plan);
    coefficient planes are overwritten (in-place IDWT).  `temp` must hold
    width*height*4 bytes (transposed scratch). */
 int BMI_plain_decode(const BmiTileDesc *tile, void *temp, size_t temp_bytes, StreamId stream);
+
+/* ------------------------------------------------------------------------
+ * Request server (continuous batching over a fixed HBM budget)
+ * --------------------------------------------------------------------- */
+
+#define BMI_SERVE_FUSED 0 /* continuous batching: waiting requests that fit
+                             are decoded together in one batch plan, several
+                             batches in flight on separate streams */
+#define BMI_SERVE_PLAIN 1 /* baseline: one request at a time through the
+                             drop-in call sequence (BMI_plain_decode) */
+
+typedef struct BmiServerConfig {
+    size_t hbm_budget;  /* bytes of HBM the server holds for requests
+                           (one allocation at create) */
+    int num_streams;    /* batches in flight at once, 1..8 (plain: 1) */
+    int max_batch;      /* requests per batch, >= 1 (plain: 1) */
+    int mode;           /* BMI_SERVE_FUSED / BMI_SERVE_PLAIN */
+    BmiTuning tuning;   /* launch shape of the batch plans */
+} BmiServerConfig;
+
+typedef struct BmiRequest {
+    /* Geometry, filter and output format of the image.  host_io = 1: coef[]
+       are host planes (pitch coef_pitch elements, pinned memory for
+       asynchronous copies) and `out` a host pixel buffer (out_pitch_px,
+       out_x, out_y), both copied by the server.  host_io = 0: they are
+       device buffers of the caller, read and written in place.
+       out_format NONE is not served. */
+    BmiTileDesc tile;
+    int host_io;
+} BmiRequest;
+
+typedef struct BmiRequestResult {
+    int status;         /* BMI_OK or the error that failed the request */
+    double submit_us;   /* server clock (us since create) at submit */
+    double start_us;    /* admitted: its batch was enqueued */
+    double done_us;     /* the server saw its batch complete */
+    int batch_id;       /* batch the request ran in */
+    int batch_size;     /* requests in that batch */
+} BmiRequestResult;
+
+typedef struct BmiServerStats {
+    long long submitted, completed, failed, batches;
+    size_t peak_bytes;   /* high-water mark of the arena */
+    int peak_inflight;   /* most batches on the device at once */
+} BmiServerStats;
+
+typedef struct BmiServer BmiServer;
+
+int BMI_server_create(BmiServer **server, const BmiServerConfig *cfg);
+/* Validate and queue one request (returns at once).  Rejects, with no
+   effect on other requests, a request that is malformed (BMI_E_INVALID /
+   BMI_E_UNSUPPORTED) or whose footprint exceeds the whole budget
+   (BMI_E_NOMEM).  *ticket identifies it for BMI_server_wait. */
+int BMI_server_submit(BmiServer *server, const BmiRequest *req, unsigned long long *ticket);
+/* Block until the request completes (timeout_ms < 0: no limit); fills
+   *result and forgets the ticket.  Returns the request's status, or
+   BMI_E_INVALID for an unknown ticket, or 1 on timeout. */
+int BMI_server_wait(BmiServer *server, unsigned long long ticket, int timeout_ms,
+                    BmiRequestResult *result);
+int BMI_server_stats(BmiServer *server, BmiServerStats *stats);
+/* Finish every queued request, then release the arena and streams. */
+int BMI_server_destroy(BmiServer *server);
 
 #ifdef __cplusplus
 }

@@ -96,6 +96,46 @@ class BmiTuning(ctypes.Structure):
     ]
 
 
+SERVE_FUSED = 0
+SERVE_PLAIN = 1
+
+
+class BmiServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget", ctypes.c_size_t),
+        ("num_streams", ctypes.c_int),
+        ("max_batch", ctypes.c_int),
+        ("mode", ctypes.c_int),
+        ("tuning", BmiTuning),
+    ]
+
+
+class BmiRequest(ctypes.Structure):
+    _fields_ = [("tile", BmiTileDesc), ("host_io", ctypes.c_int)]
+
+
+class BmiRequestResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("submit_us", ctypes.c_double),
+        ("start_us", ctypes.c_double),
+        ("done_us", ctypes.c_double),
+        ("batch_id", ctypes.c_int),
+        ("batch_size", ctypes.c_int),
+    ]
+
+
+class BmiServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_longlong),
+        ("completed", ctypes.c_longlong),
+        ("failed", ctypes.c_longlong),
+        ("batches", ctypes.c_longlong),
+        ("peak_bytes", ctypes.c_size_t),
+        ("peak_inflight", ctypes.c_int),
+    ]
+
+
 class GpuInfo(ctypes.Structure):
     _fields_ = [
         ("active", ctypes.c_int),
@@ -153,9 +193,21 @@ def lib() -> ctypes.CDLL:
     L.BMI_batch_plan_num_launches.restype = i
     L.BMI_batch_plan_execute_launches.argtypes = [vp, i, i, vp]
     L.BMI_batch_plan_execute_launches.restype = i
+    L.BMI_autotune.argtypes = [ctypes.POINTER(BmiTileDesc), i, ctypes.POINTER(BmiTuning), i, i, vp,
+                               ctypes.POINTER(BmiTuning), ctypes.POINTER(f)]
+    L.BMI_autotune.restype = i
+    L.BMI_server_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BmiServerConfig)]
+    L.BMI_server_create.restype = i
+    L.BMI_server_submit.argtypes = [vp, ctypes.POINTER(BmiRequest), ctypes.POINTER(ctypes.c_ulonglong)]
+    L.BMI_server_submit.restype = i
+    L.BMI_server_wait.argtypes = [vp, ctypes.c_ulonglong, i, ctypes.POINTER(BmiRequestResult)]
+    L.BMI_server_wait.restype = i
+    L.BMI_server_stats.argtypes = [vp, ctypes.POINTER(BmiServerStats)]
+    L.BMI_server_stats.restype = i
+    L.BMI_server_destroy.argtypes = [vp]
+    L.BMI_server_destroy.restype = i
     L.BMI_plain_decode.argtypes = [ctypes.POINTER(BmiTileDesc), vp, ctypes.c_size_t, vp]
     L.BMI_plain_decode.restype = i
-    _ = f
     _LIB = L
     return L
 
@@ -252,6 +304,21 @@ class BatchPlan:
             pass
 
 
+def autotune(descs, candidates=None, reps: int = 5, stream: int = 0):
+    """BMI_autotune: fastest tuning for these tiles -> (BmiTuning, us per execute)."""
+    arr = (BmiTileDesc * len(descs))(*descs)
+    cand = None
+    n = 0
+    if candidates:
+        cand = (BmiTuning * len(candidates))(*candidates)
+        n = len(candidates)
+    best = BmiTuning()
+    us = ctypes.c_float()
+    check(lib().BMI_autotune(arr, len(descs), cand, n, reps, stream or None, ctypes.byref(best),
+                             ctypes.byref(us)), "BMI_autotune")
+    return best, float(us.value)
+
+
 def plain_decode(desc: BmiTileDesc, temp_ptr: int, temp_bytes: int, stream: int = 0) -> None:
     """The reference's drop-in call sequence for one tile (in place)."""
     check(lib().BMI_plain_decode(ctypes.byref(desc), temp_ptr, temp_bytes, stream or None),
@@ -280,3 +347,51 @@ def output_size(w: int, h: int, x0: int, y0: int, levels: int, top_level: int = 
         return w, h
     g = tile_geometry(w, h, x0, y0, levels)[top_level - 1]
     return g["nw"], g["nh"]
+
+
+class Server:
+    """BMI_server_*: continuous-batching request server over an HBM budget."""
+
+    def __init__(self, hbm_budget: int, num_streams: int = 4, max_batch: int = 64,
+                 mode: int = SERVE_FUSED, tuning: BmiTuning | None = None):
+        cfg = BmiServerConfig()
+        cfg.hbm_budget, cfg.num_streams, cfg.max_batch, cfg.mode = (hbm_budget, num_streams,
+                                                                    max_batch, mode)
+        if tuning is not None:
+            cfg.tuning = tuning
+        self.handle = ctypes.c_void_p()
+        check(lib().BMI_server_create(ctypes.byref(self.handle), ctypes.byref(cfg)),
+              "BMI_server_create")
+
+    def submit(self, desc: BmiTileDesc, host_io: bool = False) -> int:
+        req = BmiRequest()
+        req.tile = desc
+        req.host_io = int(host_io)
+        t = ctypes.c_ulonglong()
+        check(lib().BMI_server_submit(self.handle, ctypes.byref(req), ctypes.byref(t)),
+              "BMI_server_submit")
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> BmiRequestResult:
+        res = BmiRequestResult()
+        rc = lib().BMI_server_wait(self.handle, ticket, timeout_ms, ctypes.byref(res))
+        if rc == 1:
+            raise TimeoutError(f"request {ticket} not done after {timeout_ms} ms")
+        check(rc, "BMI_server_wait")
+        return res
+
+    def stats(self) -> BmiServerStats:
+        st = BmiServerStats()
+        check(lib().BMI_server_stats(self.handle, ctypes.byref(st)), "BMI_server_stats")
+        return st
+
+    def close(self) -> None:
+        if self.handle:
+            lib().BMI_server_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -803,8 +803,10 @@ struct BmiBatchPlan {
     WorkItem *d_items = nullptr;
     int *d_counters = nullptr;  // [njobs] completion counters, [njobs] = head,
                                 // [njobs+1] = error flag
-    void *scratch = nullptr;
+    void *scratch = nullptr;     // device base of the plan's memory (scratch first)
     size_t scratch_bytes = 0;
+    size_t device_bytes = 0;     // scratch + job table + items + counters
+    bool owns_device = false;    // memory came from hipMalloc (else a caller's arena)
     std::vector<size_t> tile_scratch_off;  // per tile: base offset
     std::vector<int> tile_scratch_pitch;
     std::vector<size_t> tile_scratch_plane;  // bytes per plane
@@ -889,8 +891,14 @@ void needed_comps(const BmiTileDesc &t, int ok, int *list, int *n)
 
 static int default_seg_rows(const BmiTuning *t) { return (t && t->seg_rows > 0) ? t->seg_rows : 32; }
 
-extern "C" int BMI_batch_plan_create(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles,
-                                     const BmiTuning *tuning)
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Host half of plan construction: validates the tiles and builds the job
+// table and work items.  Device pointers into the plan's own memory
+// (scratch planes) are stored as offsets from address 0 and relocated by
+// bmi_plan_bind once the memory exists.
+int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles,
+                   const BmiTuning *tuning)
 {
     if (!out_plan || !tiles || ntiles < 1) return bmi_fail(BMI_E_INVALID, "batch_plan_create: bad args");
     *out_plan = nullptr;
@@ -943,12 +951,7 @@ extern "C" int BMI_batch_plan_create(BmiBatchPlan **out_plan, const BmiTileDesc 
         total = (total + 255) & ~(size_t)255;
     }
     P->scratch_bytes = total;
-    if (total) {
-        if (hipMalloc(&P->scratch, total) != hipSuccess) {
-            delete P;
-            return bmi_fail(BMI_E_NOMEM, "batch_plan_create: scratch of %zu bytes", total);
-        }
-    }
+    P->scratch = nullptr;  // offsets until bmi_plan_bind
 
     // jobs, ordered by level step then tile; items follow job order
     int max_steps = 0;
@@ -1132,16 +1135,65 @@ extern "C" int BMI_batch_plan_create(BmiBatchPlan **out_plan, const BmiTileDesc 
     const size_t jb = sizeof(LevelJob) * P->jobs.size();
     const size_t ib = sizeof(WorkItem) * P->items.size();
     const size_t cb = sizeof(int) * (P->jobs.size() + 3);
-    if (hipMalloc(&P->d_jobs, jb) != hipSuccess || hipMalloc(&P->d_items, ib) != hipSuccess ||
-        hipMalloc(&P->d_counters, cb) != hipSuccess) {
-        BMI_batch_plan_destroy(P);
-        return bmi_fail(BMI_E_NOMEM, "batch_plan_create: descriptor allocation");
+    P->device_bytes = align256(P->scratch_bytes) + align256(jb) + align256(ib) + align256(cb);
+    *out_plan = P;
+    return BMI_OK;
+}
+
+size_t bmi_plan_device_bytes(const BmiBatchPlan *P) { return P ? P->device_bytes : 0; }
+
+// Device half: place scratch, job table, items and counters in the
+// device_bytes at `base`, relocate the scratch pointers of the job table
+// and upload it on `stream` (asynchronous; the plan's host vectors stay
+// alive with the plan).  The counters are cleared with a memset, so a
+// caller that captures graphs must bind outside the capture.
+int bmi_plan_bind(BmiBatchPlan *P, void *base, hipStream_t stream)
+{
+    if (!P || !base || P->scratch) return bmi_fail(BMI_E_INVALID, "plan_bind: bad args");
+    char *b = (char *)base;
+    P->scratch = b;
+    const size_t jb = sizeof(LevelJob) * P->jobs.size();
+    const size_t ib = sizeof(WorkItem) * P->items.size();
+    const size_t cb = sizeof(int) * (P->jobs.size() + 3);
+    size_t off = align256(P->scratch_bytes);
+    P->d_jobs = (LevelJob *)(b + off);
+    off += align256(jb);
+    P->d_items = (WorkItem *)(b + off);
+    off += align256(ib);
+    P->d_counters = (int *)(b + off);
+    for (LevelJob &J : P->jobs) {
+        const bool packed = (J.out_kind == OUT_RGB || J.out_kind == OUT_MERGE);
+        const int nslots = packed ? 4 : 1;
+        if (J.lev > 0)
+            for (int s = 0; s < nslots; ++s) J.ll[s] = (const char *)J.ll[s] + (uintptr_t)b;
+        if (J.out_kind == OUT_SCRATCH) J.dst[0] = (char *)J.dst[0] + (uintptr_t)b;
     }
-    if (hipMemcpy(P->d_jobs, P->jobs.data(), jb, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(P->d_items, P->items.data(), ib, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(P->d_counters, 0, cb) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    if (hipMemcpyAsync(P->d_jobs, P->jobs.data(), jb, hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipMemcpyAsync(P->d_items, P->items.data(), ib, hipMemcpyHostToDevice, stream) !=
+            hipSuccess ||
+        hipMemsetAsync(P->d_counters, 0, cb, stream) != hipSuccess)
+        return bmi_fail(BMI_E_HIP, "plan_bind: descriptor upload");
+    return BMI_OK;
+}
+
+extern "C" int BMI_batch_plan_create(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles,
+                                     const BmiTuning *tuning)
+{
+    BmiBatchPlan *P = nullptr;
+    int rc = bmi_plan_build(&P, tiles, ntiles, tuning);
+    if (rc) return rc;
+    void *mem = nullptr;
+    if (hipMalloc(&mem, P->device_bytes) != hipSuccess) {
+        delete P;
+        return bmi_fail(BMI_E_NOMEM, "batch_plan_create: %zu device bytes", P->device_bytes);
+    }
+    P->owns_device = true;
+    rc = bmi_plan_bind(P, mem, nullptr);
+    if (!rc && hipStreamSynchronize(nullptr) != hipSuccess)
+        rc = bmi_fail(BMI_E_HIP, "batch_plan_create: descriptor upload");
+    if (rc) {
         BMI_batch_plan_destroy(P);
-        return bmi_fail(BMI_E_HIP, "batch_plan_create: descriptor upload");
+        return rc;
     }
     *out_plan = P;
     return BMI_OK;
@@ -1247,8 +1299,9 @@ extern "C" int BMI_batch_plan_rebind(BmiBatchPlan *P, int i, const BmiTileDesc *
     for (size_t j = 0; j < P->jobs.size(); ++j) {
         if (P->job_tile[j] != i) continue;
         LevelJob &J = P->jobs[j];
-        for (int s = 0; s < 4; ++s) {
-            if (s >= J.nc && J.nc == 1) break;
+        const bool packed = (J.out_kind == OUT_RGB || J.out_kind == OUT_MERGE);
+        for (int s = 0; s < (packed ? 4 : 1); ++s) {
+            // padded lockstep slots (s >= nc) repeat slot 0's component
             const int c = J.comp[s];
             J.det[s] = t->coef[c];
             if (J.lev == 0) J.ll[s] = t->coef[c];
@@ -1281,10 +1334,7 @@ extern "C" int BMI_batch_plan_destroy(BmiBatchPlan *P)
     if (!P) return BMI_OK;
     if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
     if (P->graph) (void)hipGraphDestroy(P->graph);
-    if (P->d_jobs) (void)hipFree(P->d_jobs);
-    if (P->d_items) (void)hipFree(P->d_items);
-    if (P->d_counters) (void)hipFree(P->d_counters);
-    if (P->scratch) (void)hipFree(P->scratch);
+    if (P->owns_device && P->scratch) (void)hipFree(P->scratch);
     delete P;
     return BMI_OK;
 }
@@ -1295,4 +1345,60 @@ extern "C" int BMI_batch_plan_error_flag(BmiBatchPlan *P)
     int v = 0;
     (void)hipMemcpy(&v, P->d_counters + P->jobs.size() + 2, sizeof(int), hipMemcpyDeviceToHost);
     return v;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: time candidate launch shapes on the caller's own tiles.
+
+extern "C" int BMI_autotune(const BmiTileDesc *tiles, int ntiles, const BmiTuning *candidates,
+                            int ncand, int reps, StreamId stream, BmiTuning *best, float *best_us)
+{
+    if (!tiles || ntiles < 1 || !best || reps < 1)
+        return bmi_fail(BMI_E_INVALID, "autotune: bad args");
+    std::vector<BmiTuning> cand;
+    if (candidates && ncand > 0) {
+        cand.assign(candidates, candidates + ncand);
+    } else {
+        // default grid: segment height x items per CU x waves per group
+        for (int seg : {16, 32, 64, 128})
+            for (int mw : {4, 8, 16})
+                for (int wpg : {2, 4}) cand.push_back(BmiTuning{seg, wpg, 0, 0, mw});
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return bmi_fail(BMI_E_HIP, "autotune: event");
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return bmi_fail(BMI_E_HIP, "autotune: event");
+    }
+    float best_t = 0.0f;
+    int best_i = -1, rc = BMI_OK;
+    for (size_t i = 0; i < cand.size() && !rc; ++i) {
+        BmiBatchPlan *P = nullptr;
+        rc = BMI_batch_plan_create(&P, tiles, ntiles, &cand[i]);
+        if (rc) break;
+        // warm-up (also first-touch of the scratch), then `reps` timed runs
+        rc = plan_enqueue(P, s);
+        if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = BMI_E_HIP;
+        for (int r = 0; r < reps && !rc; ++r) rc = plan_enqueue(P, s);
+        if (!rc && hipEventRecord(e1, s) != hipSuccess) rc = BMI_E_HIP;
+        float ms = 0.0f;
+        if (!rc && (hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = BMI_E_HIP;
+        if (!rc && BMI_batch_plan_error_flag(P) != 0) rc = BMI_E_HIP;
+        BMI_batch_plan_destroy(P);
+        if (rc) break;
+        const float us = ms * 1000.0f / reps;
+        if (best_i < 0 || us < best_t) {
+            best_t = us;
+            best_i = (int)i;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return bmi_fail(rc, "autotune: candidate failed");
+    *best = cand[best_i];
+    if (best_us) *best_us = best_t;
+    return BMI_OK;
 }

@@ -17,4 +17,15 @@ int bmi_check_hip(hipError_t e, const char *what);
 // legacy default stream
 inline hipStream_t bmi_stream(StreamId s) { return (hipStream_t)s; }
 
-extern "C" int BMI_batch_plan_error_flag(BmiBatchPlan *plan);
+
+// Plan construction in two halves, for callers that own the device memory
+// (the request server carves it from its HBM arena):
+//   bmi_plan_build      host only: validate tiles, build jobs and items
+//   bmi_plan_device_bytes  bytes the plan needs on the device
+//   bmi_plan_bind       place the plan in [base, base+bytes) and upload
+//                       its tables on `stream` (asynchronous)
+// A plan bound this way does not free its memory on destroy.
+int bmi_plan_build(BmiBatchPlan **plan, const BmiTileDesc *tiles, int ntiles,
+                   const BmiTuning *tuning);
+size_t bmi_plan_device_bytes(const BmiBatchPlan *plan);
+int bmi_plan_bind(BmiBatchPlan *plan, void *base, hipStream_t stream);
