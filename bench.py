@@ -73,6 +73,162 @@ def cpu_baseline(planes: np.ndarray, w: int, h: int, budget_s: float = 10.0):
                       f"oracle, one host thread per component ({el:.1f} s)"}
 
 
+def _device_planes(torch, dev, w, h, filt, ncomp, gen):
+    """Synthetic coefficient planes generated on the device (timing of the
+    decode does not depend on the values): 5/3 small integers, 9/7 floats
+    of the configured distributions' scale."""
+    if filt == "53":
+        return [torch.randint(-64, 64, (h, w), dtype=torch.int32, device=dev, generator=gen)
+                for _ in range(ncomp)]
+    return [(torch.randn((h, w), device=dev, generator=gen) * 8.0).view(torch.int32)
+            for _ in range(ncomp)]
+
+
+def batched_leg(pkg, wl, torch, dev, stream, steps, warmup):
+    """256 tiles (sides 64..1024, 3..6 levels, 5/3 / 9/7, 1 or 3
+    components) in one plan: autotuned launch shape, hipGraph replay;
+    beside it the plain path decoding the same tiles one per call."""
+    tiles = wl.config3_params(0)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    descs, keep = [], []
+    pixels = comp_samples = out_bytes = 0
+    for t in tiles:
+        coef = _device_planes(torch, dev, t["w"], t["h"], t["filt"], t["ncomp"], gen)
+        fmt, co = (2, -1) if t["ncomp"] == 3 else (6, 0)
+        out = torch.zeros((t["h"], t["w"] * 3), dtype=torch.uint8, device=dev)
+        descs.append(pkg.tile_desc(coef_ptrs=[c.data_ptr() for c in coef], coef_pitch=t["w"],
+                                   width=t["w"], height=t["h"], levels=t["levels"],
+                                   method=pkg.W5X3 if t["filt"] == "53" else pkg.W9X7,
+                                   ncomp=t["ncomp"], bit_depth=8, out_format=fmt, check_out=co,
+                                   out_ptr=out.data_ptr(), out_pitch_px=t["w"]))
+        keep.append((coef, out))
+        pixels += t["w"] * t["h"]
+        comp_samples += t["w"] * t["h"] * t["ncomp"]
+        out_bytes += t["w"] * t["h"] * 3
+    sh = stream.cuda_stream
+    t0 = time.perf_counter()
+    best, best_us = pkg.autotune(descs, None, reps=3, stream=sh)
+    tune_s = time.perf_counter() - t0
+    plan = pkg.BatchPlan(descs, best)
+    for _ in range(warmup):
+        plan.execute_graph(sh)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        plan.execute_graph(sh)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    compulsory = comp_samples * 4 + out_bytes
+    res = {"tiles": len(tiles), "pixels": pixels, "ms_per_batch": round(ms, 4),
+           "Mpixel_s": round(pixels / (ms * 1e-3) / 1e6, 1),
+           "compulsory_bytes": compulsory,
+           "GBs_compulsory": round(compulsory / (ms * 1e-3) / 1e9, 1),
+           "frac_of_peak": round(compulsory / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "work_items": plan.num_items, "launches": plan.num_launches,
+           "tuning": {"seg_rows": best.seg_rows, "waves_per_group": best.waves_per_group,
+                      "min_waves_per_cu": best.min_waves_per_cu, "persistent": best.persistent},
+           "autotune_s": round(tune_s, 2)}
+    plan.close()
+    # plain path: the drop-in sequence, one tile per call, on copies (in place)
+    work = [[torch.empty_like(c) for c in coef] for coef, _ in keep]
+    temps = torch.empty(1024 * 1024 + 64, dtype=torch.int32, device=dev)
+    pdescs = []
+    for d, wk in zip(descs, work):
+        pd = pkg.BmiTileDesc.from_buffer_copy(d)
+        for c, t in enumerate(wk):
+            pd.coef[c] = t.data_ptr()
+        pdescs.append(pd)
+    times = []
+    for it in range(2):
+        for wk, (coef, _) in zip(work, keep):
+            for a, b in zip(wk, coef):
+                a.copy_(b)
+        p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        p0.record(stream)
+        for pd in pdescs:
+            pkg.plain_decode(pd, temps.data_ptr(), temps.numel() * 4, sh)
+        p1.record(stream)
+        torch.cuda.synchronize()
+        if it:
+            times.append(p0.elapsed_time(p1))
+    pms = float(np.mean(times))
+    res["plain_ms_per_batch"] = round(pms, 3)
+    res["plain_Mpixel_s"] = round(pixels / (pms * 1e-3) / 1e6, 1)
+    res["speedup_vs_plain"] = round(pms / ms, 2)
+    return res
+
+
+def serving_leg(pkg, wl, torch, dev, rates, nreq, budget_gib=64):
+    """Poisson arrivals at each rate; sides log-uniform in 256..8192, 8 or
+    12 bits, 5/3 or 9/7, 3 components, 5 levels; device-resident
+    coefficient and output buffers; admitted against an HBM budget.  The
+    same arrival trace goes through the fused server and the plain server
+    (one request at a time, drop-in call sequence)."""
+    import threading
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1)
+    side = 8192
+    pools = {f: _device_planes(torch, dev, side, side, f, 3, gen) for f in ("53", "97")}
+    out = {}
+    for mode_name, mode in (("fused", pkg.SERVE_FUSED), ("plain", pkg.SERVE_PLAIN)):
+        rows = []
+        for rate in rates:
+            reqs = wl.config4_requests(7, nreq, rate)
+            outs = [torch.empty((r["h"], r["w"] * (3 if r["bits"] == 8 else 6)), dtype=torch.uint8,
+                                device=dev) for r in reqs]
+            descs = []
+            for r, o in zip(reqs, outs):
+                coef = pools[r["filt"]]
+                descs.append(pkg.tile_desc(coef_ptrs=[c.data_ptr() for c in coef], coef_pitch=side,
+                                           width=r["w"], height=r["h"], levels=5,
+                                           method=pkg.W5X3 if r["filt"] == "53" else pkg.W9X7,
+                                           ncomp=3, bit_depth=r["bits"],
+                                           out_format=2 if r["bits"] == 8 else 1,
+                                           out_ptr=o.data_ptr(), out_pitch_px=r["w"]))
+            torch.cuda.synchronize()
+            srv = pkg.Server(budget_gib << 30, num_streams=4, max_batch=32, mode=mode)
+            tickets = [None] * len(reqs)
+            results = [None] * len(reqs)
+
+            def waiter():
+                for i in range(len(reqs)):
+                    while tickets[i] is None:
+                        time.sleep(1e-4)
+                    results[i] = srv.wait(tickets[i], 120000)
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            t_start = time.perf_counter()
+            for i, r in enumerate(reqs):
+                delay = r["arrival"] - (time.perf_counter() - t_start)
+                if delay > 0:
+                    time.sleep(delay)
+                tickets[i] = srv.submit(descs[i])
+            th.join()
+            st = srv.stats()
+            srv.close()
+            lat = np.array([(x.done_us - x.submit_us) / 1e3 for x in results])
+            span = (max(x.done_us for x in results) - min(x.submit_us for x in results)) / 1e6
+            mpix = sum(r["w"] * r["h"] for r in reqs) / span / 1e6
+            rows.append({"rate_req_s": rate, "requests": len(reqs),
+                         "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                         "Mpixel_s": round(mpix, 1), "batches": int(st.batches),
+                         "mean_batch": round(len(reqs) / max(1, st.batches), 2),
+                         "failed": int(st.failed), "peak_GiB": round(st.peak_bytes / 2**30, 2)})
+            log(f"serving {mode_name} rate {rate}: p50 {rows[-1]['p50_ms']} ms, "
+                f"p99 {rows[-1]['p99_ms']} ms, {rows[-1]['Mpixel_s']} Mpixel/s")
+            del outs
+        out[mode_name] = rows
+    out["workload"] = (f"{nreq} requests per rate, Poisson arrivals, square sides log-uniform in "
+                       "256..8192, 8/12-bit, 5/3 or 9/7, 3 components, 5 levels, device-resident "
+                       f"buffers, {budget_gib} GiB budget, 4 streams, batches of at most 32")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +244,10 @@ def main() -> int:
     ap.add_argument("--waves-per-group", type=int, default=0)
     ap.add_argument("--persistent", type=int, default=0)
     ap.add_argument("--min-waves", type=int, default=0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-tile batched leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-rates", default="50,200,800,3200,12800")
+    ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 256 MiB device copy, the reference for scaling HBM counters")
     args = ap.parse_args()
@@ -221,6 +381,19 @@ def main() -> int:
                  "launches_per_step": 3 * (2 * L) + 1}
         log(f"plain: {pms * 1e3:.1f} us/step, fused speedup {pms / ms_step:.2f}x")
 
+    batched = None
+    if not args.no_batched:
+        log("batched: 256 mixed tiles, autotuned, hipGraph")
+        batched = batched_leg(pkg, wl, torch, dev, stream, max(5, args.steps // 5), 2)
+        log(f"batched: {batched['ms_per_batch']} ms, {batched['Mpixel_s']} Mpixel/s, "
+            f"{batched['frac_of_peak'] * 100:.1f}% of peak; plain {batched['plain_Mpixel_s']} "
+            f"Mpixel/s ({batched['speedup_vs_plain']}x)")
+
+    serving = None
+    if not args.no_serving:
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        serving = serving_leg(pkg, wl, torch, dev, rates, args.serve_requests)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle)")
@@ -251,6 +424,8 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_gpu": plain,
+        "batched": batched,
+        "serving": serving,
     }
     plan.close()
     print(json.dumps(rec), flush=True)

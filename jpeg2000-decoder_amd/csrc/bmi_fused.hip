@@ -1141,13 +1141,47 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
 }
 
 size_t bmi_plan_device_bytes(const BmiBatchPlan *P) { return P ? P->device_bytes : 0; }
+size_t bmi_plan_table_bytes(const BmiBatchPlan *P)
+{
+    return P ? align256(sizeof(LevelJob) * P->jobs.size()) + sizeof(WorkItem) * P->items.size() : 0;
+}
 
 // Device half: place scratch, job table, items and counters in the
 // device_bytes at `base`, relocate the scratch pointers of the job table
 // and upload it on `stream` (asynchronous; the plan's host vectors stay
 // alive with the plan).  The counters are cleared with a memset, so a
 // caller that captures graphs must bind outside the capture.
-int bmi_plan_bind(BmiBatchPlan *P, void *base, hipStream_t stream)
+size_t bmi_plan_estimate_bytes(const BmiTileDesc *t, const BmiTuning *tuning)
+{
+    (void)tuning;
+    if (!t || validate_tile(*t) != BMI_OK) return 0;
+    const int L = t->levels;
+    const int Ls = (t->top_level < 0) ? L : t->top_level;
+    int maxw = 0, maxh = 0;
+    size_t jobs = 0, items = 0;
+    for (int lev = 0; lev < Ls; ++lev) {
+        int a0, a1, b0, b1;
+        level_span(t->x0, t->width, L, lev, &a0, &a1);
+        level_span(t->y0, t->height, L, lev, &b0, &b1);
+        if (lev + 1 < Ls) {
+            maxw = std::max(maxw, a1 - a0);
+            maxh = std::max(maxh, b1 - b0);
+        }
+        const int owp = (t->method == Ckernels_W9X7) ? 60 : 62;
+        const size_t nstrips = (size_t)(a1 - (a0 & ~1) + 2 * owp - 1) / (2 * owp);
+        const size_t nrows = (size_t)(b1 - (b0 & ~1));
+        // at most one job per component; segments of one row pair at least
+        jobs += (size_t)t->ncomp;
+        items += (size_t)t->ncomp * nstrips * ((nrows + 1) / 2);
+    }
+    const int pitch = (maxw + 63) & ~63;
+    const size_t scratch = align256((size_t)pitch * maxh * 4 * 2 * (size_t)t->ncomp);
+    return scratch + align256(sizeof(LevelJob) * jobs) + align256(sizeof(WorkItem) * items) +
+           align256(sizeof(int) * (jobs + 3));
+}
+
+int bmi_plan_bind(BmiBatchPlan *P, void *base, hipStream_t stream, void *staging,
+                  size_t staging_bytes)
 {
     if (!P || !base || P->scratch) return bmi_fail(BMI_E_INVALID, "plan_bind: bad args");
     char *b = (char *)base;
@@ -1168,11 +1202,21 @@ int bmi_plan_bind(BmiBatchPlan *P, void *base, hipStream_t stream)
             for (int s = 0; s < nslots; ++s) J.ll[s] = (const char *)J.ll[s] + (uintptr_t)b;
         if (J.out_kind == OUT_SCRATCH) J.dst[0] = (char *)J.dst[0] + (uintptr_t)b;
     }
-    if (hipMemcpyAsync(P->d_jobs, P->jobs.data(), jb, hipMemcpyHostToDevice, stream) != hipSuccess ||
-        hipMemcpyAsync(P->d_items, P->items.data(), ib, hipMemcpyHostToDevice, stream) !=
-            hipSuccess ||
-        hipMemsetAsync(P->d_counters, 0, cb, stream) != hipSuccess)
+    if (staging && staging_bytes >= align256(jb) + ib) {
+        // one copy of both tables from pinned memory (same layout as on the device)
+        memcpy(staging, P->jobs.data(), jb);
+        memcpy((char *)staging + align256(jb), P->items.data(), ib);
+        if (hipMemcpyAsync(P->d_jobs, staging, align256(jb) + ib, hipMemcpyHostToDevice, stream) !=
+            hipSuccess)
+            return bmi_fail(BMI_E_HIP, "plan_bind: descriptor upload");
+    } else if (hipMemcpyAsync(P->d_jobs, P->jobs.data(), jb, hipMemcpyHostToDevice, stream) !=
+                   hipSuccess ||
+               hipMemcpyAsync(P->d_items, P->items.data(), ib, hipMemcpyHostToDevice, stream) !=
+                   hipSuccess) {
         return bmi_fail(BMI_E_HIP, "plan_bind: descriptor upload");
+    }
+    if (hipMemsetAsync(P->d_counters, 0, cb, stream) != hipSuccess)
+        return bmi_fail(BMI_E_HIP, "plan_bind: counters");
     return BMI_OK;
 }
 
@@ -1188,7 +1232,7 @@ extern "C" int BMI_batch_plan_create(BmiBatchPlan **out_plan, const BmiTileDesc 
         return bmi_fail(BMI_E_NOMEM, "batch_plan_create: %zu device bytes", P->device_bytes);
     }
     P->owns_device = true;
-    rc = bmi_plan_bind(P, mem, nullptr);
+    rc = bmi_plan_bind(P, mem, nullptr, nullptr, 0);
     if (!rc && hipStreamSynchronize(nullptr) != hipSuccess)
         rc = bmi_fail(BMI_E_HIP, "batch_plan_create: descriptor upload");
     if (rc) {

@@ -23,9 +23,17 @@ inline hipStream_t bmi_stream(StreamId s) { return (hipStream_t)s; }
 //   bmi_plan_build      host only: validate tiles, build jobs and items
 //   bmi_plan_device_bytes  bytes the plan needs on the device
 //   bmi_plan_bind       place the plan in [base, base+bytes) and upload
-//                       its tables on `stream` (asynchronous)
+//                       its tables on `stream` (asynchronous); with a
+//                       pinned `staging` buffer of at least
+//                       bmi_plan_table_bytes, one copy from it (the staging
+//                       must stay untouched until the stream reaches it)
+//   bmi_plan_estimate_bytes  upper bound of device bytes of a one-tile
+//                       plan, without building it (0 for an invalid tile)
 // A plan bound this way does not free its memory on destroy.
 int bmi_plan_build(BmiBatchPlan **plan, const BmiTileDesc *tiles, int ntiles,
                    const BmiTuning *tuning);
 size_t bmi_plan_device_bytes(const BmiBatchPlan *plan);
-int bmi_plan_bind(BmiBatchPlan *plan, void *base, hipStream_t stream);
+size_t bmi_plan_table_bytes(const BmiBatchPlan *plan);
+size_t bmi_plan_estimate_bytes(const BmiTileDesc *tile, const BmiTuning *tuning);
+int bmi_plan_bind(BmiBatchPlan *plan, void *base, hipStream_t stream, void *staging,
+                  size_t staging_bytes);

@@ -97,6 +97,8 @@ struct BmiServer {
     Arena arena;
     std::vector<hipStream_t> streams;
     std::vector<bool> stream_busy;
+    std::vector<void *> staging;        // per stream: pinned copy of plan tables
+    std::vector<size_t> staging_bytes;
     std::chrono::steady_clock::time_point t0;
 
     std::mutex mu;
@@ -132,49 +134,33 @@ bool BmiServer::launch(std::vector<Req *> &set)
     hipStream_t s = streams[si];
     const bool plain = cfg.mode == BMI_SERVE_PLAIN;
 
-    // device descriptors of the set; coefficient / output copies follow the
-    // plan memory in the batch's block
-    std::vector<BmiTileDesc> d(set.size());
-    size_t io = 0;
-    for (size_t i = 0; i < set.size(); ++i) {
-        d[i] = set[i]->r.tile;
-        io += set[i]->io_bytes;
+    // The batch's block holds the requests' copies first, then the plan.
+    // Its size is the sum of the admitted footprints (plan part: the
+    // per-request upper bounds), so the plan built below always fits.
+    size_t io = 0, plan_res = 0;
+    for (Req *q : set) {
+        io += q->io_bytes;
+        plan_res += q->plan_bytes;
     }
-    BmiBatchPlan *plan = nullptr;
-    size_t plan_bytes = 0;
-    if (!plain) {
-        if (bmi_plan_build(&plan, d.data(), (int)d.size(), &cfg.tuning) != BMI_OK) {
-            // cannot happen for validated requests; fail the set, keep serving
-            Batch b;
-            b.id = next_batch++;
-            b.reqs = set;
-            complete(b, BMI_E_INVALID);
-            return true;
-        }
-        plan_bytes = al(bmi_plan_device_bytes(plan));
-    }
-    const size_t need = plan_bytes + io;
-    const size_t off = arena.alloc(need, kAlign);
-    if (off == Arena::kNone) {
-        if (plan) BMI_batch_plan_destroy(plan);
-        return false;
-    }
+    const size_t off = arena.alloc(io + plan_res, kAlign);
+    if (off == Arena::kNone) return false;
     Batch b;
     b.id = next_batch++;
     b.stream = si;
     b.reqs = set;
-    b.plan = plan;
     b.off = off;
     char *blk = (char *)base + off;
     const double t_start = now_us(t0);
     int rc = BMI_OK;
 
     // carve per-request copies and rewrite the descriptors to device buffers
-    char *p = blk + plan_bytes;
+    std::vector<BmiTileDesc> d(set.size());
+    char *p = blk;
     std::vector<void *> temp(set.size(), nullptr);
     for (size_t i = 0; i < set.size() && !rc; ++i) {
         Req &q = *set[i];
         BmiTileDesc &t = d[i];
+        t = q.r.tile;
         const BmiTileDesc &src = q.r.tile;
         const size_t plane = (size_t)q.dev_pitch * src.height * 4;
         if (q.r.host_io || plain) {
@@ -208,15 +194,22 @@ bool BmiServer::launch(std::vector<Req *> &set)
                                       (size_t)set[i]->r.tile.width * set[i]->r.tile.height * 4 + 256,
                                       s);
         } else {
-            // the plan was built on the caller's descriptors: rebuild it on
-            // the device ones (same geometry, so the same bytes)
-            BMI_batch_plan_destroy(plan);
-            plan = nullptr;
-            b.plan = nullptr;
+            BmiBatchPlan *plan = nullptr;
             rc = bmi_plan_build(&plan, d.data(), (int)d.size(), &cfg.tuning);
             b.plan = plan;
-            if (!rc && al(bmi_plan_device_bytes(plan)) != plan_bytes) rc = BMI_E_INVALID;
-            if (!rc) rc = bmi_plan_bind(plan, blk, s);
+            if (!rc && bmi_plan_device_bytes(plan) > plan_res) rc = BMI_E_NOMEM;
+            if (!rc) {
+                // pinned staging of this stream (free: the stream is idle)
+                const size_t tb = bmi_plan_table_bytes(plan);
+                if (staging_bytes[si] < tb) {
+                    if (staging[si]) (void)hipHostFree(staging[si]);
+                    staging[si] = nullptr;
+                    staging_bytes[si] = 0;
+                    const size_t want = std::max(tb + tb / 2, (size_t)1 << 20);
+                    if (hipHostMalloc(&staging[si], want, 0) == hipSuccess) staging_bytes[si] = want;
+                }
+                rc = bmi_plan_bind(plan, p, s, staging[si], staging_bytes[si]);
+            }
             if (!rc) rc = BMI_batch_plan_execute(plan, s);
         }
     }
@@ -370,6 +363,8 @@ extern "C" int BMI_server_create(BmiServer **out, const BmiServerConfig *cfg)
         }
         S->streams.push_back(s);
         S->stream_busy.push_back(false);
+        S->staging.push_back(nullptr);
+        S->staging_bytes.push_back(0);
     }
     S->t0 = std::chrono::steady_clock::now();
     S->thread = std::thread([S] { S->run(); });
@@ -384,20 +379,21 @@ extern "C" int BMI_server_submit(BmiServer *S, const BmiRequest *req, unsigned l
     if (t.out_format == IMAGE_FOTMAT_NONE || pixel_bytes(t.out_format) == 0)
         return bmi_fail(BMI_E_UNSUPPORTED, "server_submit: output format %d not served",
                         t.out_format);
+    const double t_submit = now_us(S->t0);
+    // validates the tile (0 = rejected) and bounds its plan's device bytes
+    const size_t est = bmi_plan_estimate_bytes(&t, &S->cfg.tuning);
+    if (!est) {
+        BmiBatchPlan *P = nullptr;  // for the precise error code
+        int rc = bmi_plan_build(&P, &t, 1, &S->cfg.tuning);
+        if (P) BMI_batch_plan_destroy(P);
+        return bmi_fail(rc ? rc : BMI_E_INVALID, "server_submit: request rejected");
+    }
     Req *q = new Req();
     q->r = *req;
     output_size(t, &q->ow, &q->oh);
     q->dev_pitch = (t.width + 63) & ~63;
-    // single-request plan: validates the tile and prices its scratch
-    BmiBatchPlan *P = nullptr;
-    int rc = bmi_plan_build(&P, &t, 1, &S->cfg.tuning);
-    if (rc) {
-        delete q;
-        return bmi_fail(rc, "server_submit: request rejected");
-    }
     const bool plain = S->cfg.mode == BMI_SERVE_PLAIN;
-    q->plan_bytes = plain ? 0 : al(bmi_plan_device_bytes(P));
-    BMI_batch_plan_destroy(P);
+    q->plan_bytes = plain ? 0 : al(est);
     const size_t plane = al((size_t)q->dev_pitch * t.height * 4);
     if (req->host_io || plain) q->io_bytes += plane * t.ncomp;
     if (req->host_io) q->io_bytes += al((size_t)q->ow * q->oh * pixel_bytes(t.out_format));
@@ -414,7 +410,7 @@ extern "C" int BMI_server_submit(BmiServer *S, const BmiRequest *req, unsigned l
             return bmi_fail(BMI_E_INVALID, "server_submit: server is stopping");
         }
         q->ticket = S->next_ticket++;
-        q->res.submit_us = now_us(S->t0);
+        q->res.submit_us = t_submit;
         S->reqs[q->ticket] = q;
         S->pending.push_back(q);
         S->stats.submitted++;
@@ -462,6 +458,8 @@ extern "C" int BMI_server_destroy(BmiServer *S)
     S->cv_work.notify_all();
     if (S->thread.joinable()) S->thread.join();
     for (hipStream_t s : S->streams) (void)hipStreamDestroy(s);
+    for (void *h : S->staging)
+        if (h) (void)hipHostFree(h);
     if (S->base) (void)hipFree(S->base);
     for (auto &kv : S->reqs) delete kv.second;
     delete S;
