@@ -136,6 +136,28 @@ class BmiServerStats(ctypes.Structure):
     ]
 
 
+T1_WORDS = 0
+T1_INT = 1
+
+
+class BmiT1Block(ctypes.Structure):
+    """struct BmiT1Block (include/bmi_t1.h)."""
+
+    _fields_ = [
+        ("data_offset", ctypes.c_ulonglong),
+        ("len", ctypes.c_int),
+        ("width", ctypes.c_int),
+        ("height", ctypes.c_int),
+        ("orient", ctypes.c_int),
+        ("numbps", ctypes.c_int),
+        ("missing", ctypes.c_int),
+        ("num_pass", ctypes.c_int),
+        ("kind", ctypes.c_int),
+        ("out", ctypes.c_void_p),
+        ("out_pitch", ctypes.c_int),
+    ]
+
+
 class GpuInfo(ctypes.Structure):
     _fields_ = [
         ("active", ctypes.c_int),
@@ -161,6 +183,15 @@ def lib() -> ctypes.CDLL:
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with `make -C {PKG_DIR}` "
             "(or __graft_entry__.build()); there is no CPU fallback")
+    # The library needs the HIP runtime (NEEDED libamdhip64.so.7).  torch
+    # bundles its own copy of that runtime; loading torch first makes the
+    # library bind to the copy torch already uses, so one process never runs
+    # two HIP runtimes side by side (device buffers and streams are shared
+    # with torch).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(str(LIB_PATH))
     vp, i, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
     L.BMI_last_error.restype = i
@@ -206,6 +237,10 @@ def lib() -> ctypes.CDLL:
     L.BMI_server_stats.restype = i
     L.BMI_server_destroy.argtypes = [vp]
     L.BMI_server_destroy.restype = i
+    L.BMI_t1_decode_blocks.argtypes = [vp, i, vp, vp]
+    L.BMI_t1_decode_blocks.restype = i
+    L.BMI_t1_check_blocks.argtypes = [ctypes.POINTER(BmiT1Block), i, ctypes.POINTER(i)]
+    L.BMI_t1_check_blocks.restype = i
     L.BMI_plain_decode.argtypes = [ctypes.POINTER(BmiTileDesc), vp, ctypes.c_size_t, vp]
     L.BMI_plain_decode.restype = i
     _LIB = L
@@ -395,3 +430,18 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def t1_check_blocks(blocks) -> None:
+    """BMI_t1_check_blocks on a host array of BmiT1Block."""
+    arr = (BmiT1Block * len(blocks))(*blocks)
+    bad = ctypes.c_int(-1)
+    rc = lib().BMI_t1_check_blocks(arr, len(blocks), ctypes.byref(bad))
+    if rc != BMI_OK:
+        raise BmiError(f"BMI_t1_check_blocks: block {bad.value}: {lib().BMI_error_string(rc).decode()}")
+
+
+def t1_decode_blocks(blocks_dev_ptr: int, nblocks: int, data_dev_ptr: int, stream: int = 0) -> None:
+    """BMI_t1_decode_blocks on device block table and code-stream buffer."""
+    check(lib().BMI_t1_decode_blocks(blocks_dev_ptr, nblocks, data_dev_ptr, stream or None),
+          "BMI_t1_decode_blocks")

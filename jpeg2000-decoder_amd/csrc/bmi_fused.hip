@@ -1249,6 +1249,7 @@ static int plan_enqueue(BmiBatchPlan *P, hipStream_t s)
     const int n = (int)P->items.size();
     if (n == 0) return BMI_OK;
     const int wpg = P->tuning.waves_per_group;
+    (void)hipGetLastError();  // report this plan's launches, not an earlier error
     if (P->tuning.persistent) {
         const int max_groups = P->num_cus * P->tuning.groups_per_cu;
         const int need_groups = (n + wpg - 1) / wpg;

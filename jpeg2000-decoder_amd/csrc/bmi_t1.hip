@@ -1,0 +1,614 @@
+// bmi_t1.hip -- EBCOT tier-1 code-block decoding on gfx950, one code-block
+// per wavefront.
+//
+// The MQ decoder is one serial chain of decisions per code-block, so a
+// wave runs it as wave-uniform code: every value of the chain (MQ
+// registers, context states, the current sample's state word) is uniform
+// and lives in scalar registers, and the branches are scalar.  The 64
+// lanes do the parallel parts: clearing the block state, and writing the
+// decoded block (one lane per column, coalesced rows).
+//
+// Per wave, LDS holds the sample words (up to 64x64 x 4 B) and one 16-bit
+// state word per sample on a grid with a one-sample border.  The state word
+// carries the significance of the eight neighbours, so a context is formed
+// from a single LDS read:
+//   bits 0..7   significance of NW, N, NE, W, E, SW, S, SE
+//   bit 8       significant        bit 9   negative
+//   bit 10      coded in this plane's significance pass
+//   bit 11      refinable (significant before the current plane)
+//   bits 12..15 sign of the N, S, W, E neighbours
+//
+// Semantics follow decode_code_block (decoder/codec/src/bmi_block_decoder.cpp:
+// 589-853) and its passes (:10-586), the MQ decoder
+// (decoder/codec/src/bmi_mq_decoder.cpp:186-355) and dec_cb_stripe's copy
+// (decoder/codec/src/dec_t1_comp.cpp:283-426); the procedures are written
+// from ITU-T T.800 C.3 and D.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bmi_device.h"
+#include "bmi_internal.h"
+#include "bmi_t1.h"
+
+namespace bmi {
+namespace {
+
+constexpr int kMaxSide = 64;
+constexpr int kWavesPerGroup = 2;
+
+// T.800 Table C.2 packed: Qe | NMPS << 16 | NLPS << 22 | SWITCH << 28
+#define MQE(q, nm, nl, sw) ((uint32_t)(q) | ((uint32_t)(nm) << 16) | ((uint32_t)(nl) << 22) | ((uint32_t)(sw) << 28))
+__constant__ uint32_t kMqTable[47] = {
+    MQE(0x5601, 1, 1, 1),    MQE(0x3401, 2, 6, 0),    MQE(0x1801, 3, 9, 0),
+    MQE(0x0AC1, 4, 12, 0),   MQE(0x0521, 5, 29, 0),   MQE(0x0221, 38, 33, 0),
+    MQE(0x5601, 7, 6, 1),    MQE(0x5401, 8, 14, 0),   MQE(0x4801, 9, 14, 0),
+    MQE(0x3801, 10, 14, 0),  MQE(0x3001, 11, 17, 0),  MQE(0x2401, 12, 18, 0),
+    MQE(0x1C01, 13, 20, 0),  MQE(0x1601, 29, 21, 0),  MQE(0x5601, 15, 14, 1),
+    MQE(0x5401, 16, 14, 0),  MQE(0x5101, 17, 15, 0),  MQE(0x4801, 18, 16, 0),
+    MQE(0x3801, 19, 17, 0),  MQE(0x3401, 20, 18, 0),  MQE(0x3001, 21, 19, 0),
+    MQE(0x2801, 22, 19, 0),  MQE(0x2401, 23, 20, 0),  MQE(0x2201, 24, 21, 0),
+    MQE(0x1C01, 25, 22, 0),  MQE(0x1801, 26, 23, 0),  MQE(0x1601, 27, 24, 0),
+    MQE(0x1401, 28, 25, 0),  MQE(0x1201, 29, 26, 0),  MQE(0x1101, 30, 27, 0),
+    MQE(0x0AC1, 31, 28, 0),  MQE(0x09C1, 32, 29, 0),  MQE(0x08A1, 33, 30, 0),
+    MQE(0x0521, 34, 31, 0),  MQE(0x0441, 35, 32, 0),  MQE(0x02A1, 36, 33, 0),
+    MQE(0x0221, 37, 34, 0),  MQE(0x0141, 38, 35, 0),  MQE(0x0111, 39, 36, 0),
+    MQE(0x0085, 40, 37, 0),  MQE(0x0049, 41, 38, 0),  MQE(0x0025, 42, 39, 0),
+    MQE(0x0015, 43, 40, 0),  MQE(0x0009, 44, 41, 0),  MQE(0x0005, 45, 42, 0),
+    MQE(0x0001, 45, 43, 0),  MQE(0x5601, 46, 46, 0)};
+#undef MQE
+
+// state word bits
+constexpr uint32_t S_NW = 1u << 0, S_N = 1u << 1, S_NE = 1u << 2, S_W = 1u << 3, S_E = 1u << 4,
+                   S_SW = 1u << 5, S_S = 1u << 6, S_SE = 1u << 7;
+constexpr uint32_t S_NB = 0xFFu;
+constexpr uint32_t S_SIG = 1u << 8, S_NEG = 1u << 9, S_VISIT = 1u << 10, S_MU = 1u << 11;
+constexpr uint32_t S_NNEG = 1u << 12, S_SNEG = 1u << 13, S_WNEG = 1u << 14, S_ENEG = 1u << 15;
+
+// contexts: 0..8 zero coding, 9..13 sign, 14..16 refinement, 17 run, 18 uniform
+constexpr int CX_SC = 9, CX_MR = 14, CX_RL = 17, CX_UNI = 18;
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// zero-coding context from the neighbour bits (T.800 Table D.1)
+__device__ __forceinline__ uint32_t zc_context(uint32_t s, int orient)
+{
+    const uint32_t h = __builtin_popcount(s & (S_W | S_E));
+    const uint32_t v = __builtin_popcount(s & (S_N | S_S));
+    const uint32_t d = __builtin_popcount(s & (S_NW | S_NE | S_SW | S_SE));
+    if (orient == 3) {
+        const uint32_t hv = h + v;
+        if (d >= 3) return 8;
+        if (d == 2) return hv ? 7 : 6;
+        if (d == 1) return hv >= 2 ? 5 : 3 + hv;
+        return hv >= 2 ? 2 : hv;
+    }
+    const uint32_t a = orient == 1 ? v : h;
+    const uint32_t b = orient == 1 ? h : v;
+    if (a == 2) return 8;
+    if (a == 1) return b ? 7 : (d ? 6 : 5);
+    if (b) return b == 2 ? 4 : 3;
+    return d >= 2 ? 2 : d;
+}
+
+// sign context (T.800 Table D.3): returns context | xor << 8
+__device__ __forceinline__ uint32_t sc_context(uint32_t s)
+{
+    const int w = (s & S_W) ? ((s & S_WNEG) ? -1 : 1) : 0;
+    const int e = (s & S_E) ? ((s & S_ENEG) ? -1 : 1) : 0;
+    const int n = (s & S_N) ? ((s & S_NNEG) ? -1 : 1) : 0;
+    const int so = (s & S_S) ? ((s & S_SNEG) ? -1 : 1) : 0;
+    int h = w + e, v = n + so;
+    h = h > 0 ? 1 : (h < 0 ? -1 : 0);
+    v = v > 0 ? 1 : (v < 0 ? -1 : 0);
+    uint32_t xr = 0;
+    if (h < 0 || (h == 0 && v < 0)) {
+        h = -h;
+        v = -v;
+        xr = 1;
+    }
+    return (uint32_t)(CX_SC + (h == 0 ? v : 3 + v)) | (xr << 8);
+}
+
+constexpr int kMaxStripes = kMaxSide / 4;
+
+// Per wave: sample words and state words of the whole block, stripe-major
+// and column-major inside a stripe, so one lane loads one column's four
+// rows with one access.
+struct WaveLds {
+    uint32_t val[kMaxStripes][kMaxSide][4];
+    uint32_t st[kMaxStripes][kMaxSide][2];  // 16-bit state of rows 0,1 | 2,3
+};
+
+__device__ __forceinline__ uint32_t rdl(uint32_t v, uint32_t lane)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+// value v with lane `lane` replaced by the uniform s (a compare and a
+// select per lane)
+__device__ __forceinline__ uint32_t wrl(uint32_t s, uint32_t lane, uint32_t v)
+{
+    return ((threadIdx.x & 63) == lane) ? s : v;
+}
+
+// MQ decoder (T.800 C.3).  Registers A, C, CT and the byte position are
+// wave-uniform (scalar); the Qe table, the 19 context states and a
+// 2 x 256-byte window of the code stream live one entry per lane in vector
+// registers and are read and written with readlane / writelane.
+struct Mq {
+    uint32_t A, C;
+    int CT;
+    int bp;       // segment index of the current byte
+    int len;
+    int mis;      // segment start modulo 4
+    uintptr_t abase;  // segment start rounded down to 4 bytes
+    uintptr_t aend;   // first address past the segment
+    int chunk;    // highest 256-byte chunk accessed so far
+    uint32_t win0, win1, win2, win3;  // per lane: dword `lane` of chunk k in slot k & 3
+    uint32_t qtab;        // per lane: Table C.2 entry `lane`
+    uint32_t ctx;         // per lane: state (index << 1 | MPS) of context `lane`
+
+    __device__ __forceinline__ uint32_t load_chunk(int k, uint32_t lane) const
+    {
+        const uintptr_t a = abase + (uintptr_t)k * 256 + 4 * lane;
+        return a < aend ? *gptr<uint32_t>((const void *)a) : 0xFFFFFFFFu;
+    }
+    __device__ __forceinline__ uint32_t byte_at(int i, uint32_t lane)
+    {
+        if (i >= len) return 0xFF;
+        const int j = i + mis;
+        const int k = j >> 8;
+        if (k > chunk) {
+            // entering chunk k (byte accesses move one byte at a time, so
+            // k = chunk + 1): prefetch chunk k + 2 into the slot of chunk
+            // k - 2, which no access can reach any more (accesses are at
+            // the current byte or the one after it)
+            chunk = k;
+            const uint32_t nv = load_chunk(k + 2, lane);
+            switch ((k + 2) & 3) {
+            case 0: win0 = nv; break;
+            case 1: win1 = nv; break;
+            case 2: win2 = nv; break;
+            default: win3 = nv; break;
+            }
+        }
+        const uint32_t l = (j >> 2) & 63;
+        uint32_t word;
+        switch (k & 3) {
+        case 0: word = rdl(win0, l); break;
+        case 1: word = rdl(win1, l); break;
+        case 2: word = rdl(win2, l); break;
+        default: word = rdl(win3, l); break;
+        }
+        return (word >> ((j & 3) * 8)) & 0xFF;
+    }
+    __device__ __forceinline__ void bytein(uint32_t lane)
+    {
+        if (byte_at(bp, lane) == 0xFF) {
+            const uint32_t b1 = byte_at(bp + 1, lane);
+            if (b1 > 0x8F) {
+                C += 0xFF00;
+                CT = 8;
+            } else {
+                ++bp;
+                C += b1 << 9;
+                CT = 7;
+            }
+        } else {
+            ++bp;
+            C += byte_at(bp, lane) << 8;
+            CT = 8;
+        }
+    }
+    __device__ __forceinline__ void init(const uint8_t *seg, int n, uint32_t lane)
+    {
+        len = n;
+        const uintptr_t sa = (uintptr_t)seg;
+        mis = (int)(sa & 3);
+        abase = sa & ~(uintptr_t)3;
+        aend = sa + (uintptr_t)n;
+        win0 = load_chunk(0, lane);
+        win1 = load_chunk(1, lane);
+        win2 = load_chunk(2, lane);
+        win3 = 0xFFFFFFFFu;
+        chunk = 0;
+        qtab = lane < 47 ? kMqTable[lane] : 0u;
+        // all contexts index 0 / MPS 0, except zero coding 0 (index 4),
+        // run (index 3) and uniform (index 46)
+        ctx = lane == 0 ? (4u << 1) : (lane == CX_RL ? (3u << 1) : (lane == CX_UNI ? (46u << 1) : 0u));
+        bp = 0;
+        C = byte_at(0, lane) << 16;
+        bytein(lane);
+        C <<= 7;
+        CT -= 7;
+        A = 0x8000;
+    }
+    __device__ __forceinline__ void renorm(uint32_t lane)
+    {
+        do {
+            if (CT == 0) bytein(lane);
+            A <<= 1;
+            C <<= 1;
+            --CT;
+        } while (!(A & 0x8000));
+    }
+    __device__ __forceinline__ uint32_t decode(uint32_t cx, uint32_t lane)
+    {
+        const uint32_t s = rdl(ctx, cx);
+        const uint32_t idx = s >> 1, mps = s & 1;
+        const uint32_t e = rdl(qtab, idx);
+        const uint32_t qe = e & 0xFFFF;
+        uint32_t d, ns;
+        A -= qe;
+        if ((C >> 16) < qe) {
+            if (A < qe) {
+                d = mps;
+                ns = (((e >> 16) & 63) << 1) | mps;
+            } else {
+                d = mps ^ 1;
+                ns = (((e >> 22) & 63) << 1) | (mps ^ (e >> 28));
+            }
+            ctx = wrl(ns, cx, ctx);
+            A = qe;
+            renorm(lane);
+        } else {
+            C -= qe << 16;
+            if (!(A & 0x8000)) {
+                if (A < qe) {
+                    d = mps ^ 1;
+                    ns = (((e >> 22) & 63) << 1) | (mps ^ (e >> 28));
+                } else {
+                    d = mps;
+                    ns = (((e >> 16) & 63) << 1) | mps;
+                }
+                ctx = wrl(ns, cx, ctx);
+                renorm(lane);
+            } else {
+                d = mps;
+            }
+        }
+        return d;
+    }
+};
+
+// State of the stripe being decoded: lane x holds column x (rows 0,1 in
+// lo, rows 2,3 in hi, 16 bits each) and its four sample words; pending
+// neighbour bits for row 3 of the stripe above and row 0 of the stripe
+// below, applied when those stripes are next touched.
+struct Stripe {
+    uint32_t lo, hi;
+    uint32_t v0, v1, v2, v3;
+    uint32_t pend_prev;  // hi-word bits (row 3) of the stripe above
+    uint32_t pend_next;  // lo-word bits (row 0) of the stripe below
+};
+
+template <int R>
+__device__ __forceinline__ uint32_t row_state(uint32_t lo, uint32_t hi)
+{
+    return ((R < 2 ? lo : hi) >> (16 * (R & 1))) & 0xFFFF;
+}
+
+template <int R>
+__device__ __forceinline__ uint32_t &vrow(Stripe &S)
+{
+    return R == 0 ? S.v0 : (R == 1 ? S.v1 : (R == 2 ? S.v2 : S.v3));
+}
+
+// OR 16-bit row bits `m` into row `r` of the (lo, hi) pair
+__device__ __forceinline__ void or_row(uint32_t &lo, uint32_t &hi, int r, uint32_t m)
+{
+    if (r < 2)
+        lo |= m << (16 * (r & 1));
+    else
+        hi |= m << (16 * (r & 1));
+}
+
+// sample (x, R) became significant with sign `neg`: update the state of
+// its own column (uniform lo/hi), of columns x-1 / x+1 (in the vectors)
+// and the pending bits of the stripes above / below
+template <int R>
+__device__ __forceinline__ void significant(Stripe &S, uint32_t &lo, uint32_t &hi, int x, int w,
+                                            uint32_t neg)
+{
+    or_row(lo, hi, R, S_SIG | (neg ? S_NEG : 0));
+    const uint32_t vn = S_N | (neg ? S_NNEG : 0), vs = S_S | (neg ? S_SNEG : 0);
+    if (R > 0) or_row(lo, hi, R - 1, vs);
+    if (R < 3) or_row(lo, hi, R + 1, vn);
+    if (R == 0) S.pend_prev = wrl(rdl(S.pend_prev, x) | (vs << 16), x, S.pend_prev);
+    if (R == 3) S.pend_next = wrl(rdl(S.pend_next, x) | vn, x, S.pend_next);
+    // the columns to the left and right
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const int c = side ? x + 1 : x - 1;
+        if (c < 0 || c >= w) continue;
+        const uint32_t hbit = side ? (S_W | (neg ? S_WNEG : 0)) : (S_E | (neg ? S_ENEG : 0));
+        const uint32_t up = side ? S_SW : S_SE;    // for the row above
+        const uint32_t down = side ? S_NW : S_NE;  // for the row below
+        uint32_t clo = 0, chi = 0;
+        or_row(clo, chi, R, hbit);
+        if (R > 0) or_row(clo, chi, R - 1, up);
+        if (R < 3) or_row(clo, chi, R + 1, down);
+        if (clo) S.lo = wrl(rdl(S.lo, c) | clo, c, S.lo);
+        if (chi) S.hi = wrl(rdl(S.hi, c) | chi, c, S.hi);
+        if (R == 0) S.pend_prev = wrl(rdl(S.pend_prev, c) | (up << 16), c, S.pend_prev);
+        if (R == 3) S.pend_next = wrl(rdl(S.pend_next, c) | down, c, S.pend_next);
+    }
+}
+
+// significance propagation of row R of column x
+template <int R>
+__device__ __forceinline__ void sigprop_row(Mq &m, Stripe &S, uint32_t &lo, uint32_t &hi, int x,
+                                            int w, int rows, int orient, uint32_t one_half,
+                                            uint32_t lane)
+{
+    if (R >= rows) return;
+    const uint32_t s = row_state<R>(lo, hi);
+    if ((s & S_SIG) || !(s & S_NB)) return;
+    or_row(lo, hi, R, S_VISIT);
+    if (!m.decode(zc_context(s, orient), lane)) return;
+    const uint32_t sc = sc_context(s);
+    const uint32_t neg = m.decode(sc & 0xFF, lane) ^ (sc >> 8);
+    significant<R>(S, lo, hi, x, w, neg);
+    vrow<R>(S) = wrl((neg << 31) + one_half, x, vrow<R>(S));
+}
+
+template <int R>
+__device__ __forceinline__ void magref_row(Mq &m, Stripe &S, uint32_t lo, uint32_t hi, int x,
+                                           int rows, int plane, uint32_t half, uint32_t lane)
+{
+    if (R >= rows) return;
+    const uint32_t s = row_state<R>(lo, hi);
+    if (!(s & S_MU)) return;
+    const uint32_t v = rdl(vrow<R>(S), x);
+    const uint32_t mag = (v & 0x7FFFFFFFu) >> plane;
+    const uint32_t cx = mag < 4 ? CX_MR + ((s & S_NB) ? 1 : 0) : CX_MR + 2;
+    const uint32_t bit = m.decode(cx, lane);
+    vrow<R>(S) = wrl((v ^ ((1u - bit) << plane)) | half, x, vrow<R>(S));
+}
+
+// cleanup of row R of column x; rows before `first` are skipped (run mode)
+template <int R>
+__device__ __forceinline__ void cleanup_row(Mq &m, Stripe &S, uint32_t &lo, uint32_t &hi, int x,
+                                            int w, int rows, int orient, uint32_t one_half,
+                                            int first, uint32_t lane)
+{
+    if (R >= rows || R < first) return;
+    const uint32_t s = row_state<R>(lo, hi);
+    if (s & (S_SIG | S_VISIT)) return;
+    if (!m.decode(zc_context(s, orient), lane)) return;
+    const uint32_t sc = sc_context(s);
+    const uint32_t neg = m.decode(sc & 0xFF, lane) ^ (sc >> 8);
+    significant<R>(S, lo, hi, x, w, neg);
+    vrow<R>(S) = wrl((neg << 31) + one_half, x, vrow<R>(S));
+}
+
+template <int R>
+__device__ __forceinline__ void run_hit(Mq &m, Stripe &S, uint32_t &lo, uint32_t &hi, int x, int w,
+                                        uint32_t one_half, uint32_t lane)
+{
+    const uint32_t s = row_state<R>(lo, hi);
+    const uint32_t sc = sc_context(s);
+    const uint32_t neg = m.decode(sc & 0xFF, lane) ^ (sc >> 8);
+    significant<R>(S, lo, hi, x, w, neg);
+    vrow<R>(S) = wrl((neg << 31) + one_half, x, vrow<R>(S));
+}
+
+// mask of the 16-bit rows [0, rows) in the (lo, hi) pair, as (mlo, mhi)
+__device__ __forceinline__ void rows_mask(int rows, uint32_t bits, uint32_t &mlo, uint32_t &mhi)
+{
+    mlo = (rows > 0 ? bits : 0) | (rows > 1 ? bits << 16 : 0);
+    mhi = (rows > 2 ? bits : 0) | (rows > 3 ? bits << 16 : 0);
+}
+
+__global__ void __launch_bounds__(64 * kWavesPerGroup)
+    t1_decode_kernel(const BmiT1Block *__restrict__ blocks, int nblocks,
+                     const uint8_t *__restrict__ data)
+{
+    __shared__ WaveLds lds_all[kWavesPerGroup];
+    const uint32_t lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int bi = (int)uni(blockIdx.x * kWavesPerGroup + wave);
+    if (bi >= nblocks) return;
+    WaveLds &L = lds_all[wave];
+    const BmiT1Block *B = blocks + bi;
+    const int w = (int)uni(B->width), h = (int)uni(B->height);
+    if (w < 1 || h < 1 || w > kMaxSide || h > kMaxSide) return;
+    const int orient = (int)uni(B->orient);
+    const int numbps = (int)uni(B->numbps);
+    const int missing = (int)uni(B->missing);
+    const int num_pass = (int)uni(B->num_pass);
+    const int kind = (int)uni(B->kind);
+    const int len = (int)uni(B->len);
+    const uint64_t off = ((uint64_t)uni((uint32_t)(B->data_offset >> 32)) << 32) |
+                         uni((uint32_t)B->data_offset);
+    const uintptr_t outp = ((uintptr_t)uni((uint32_t)((uintptr_t)B->out >> 32)) << 32) |
+                           uni((uint32_t)(uintptr_t)B->out);
+    const int out_pitch = (int)uni(B->out_pitch);
+    const int nstripes = (h + 3) >> 2;
+    const uint64_t colmask = (w == 64) ? ~0ull : ((1ull << w) - 1);
+
+    for (int s = 0; s < nstripes; ++s) {
+        L.val[s][lane][0] = L.val[s][lane][1] = L.val[s][lane][2] = L.val[s][lane][3] = 0;
+        L.st[s][lane][0] = L.st[s][lane][1] = 0;
+    }
+
+    // decode_code_block :610-649
+    int num_passes = 0;
+    int plane = 0;
+    if ((numbps - missing) * 3 - 2 > 0) {
+        const int p_max = 30 - missing;
+        num_passes = 3 * p_max - 2;
+        if (num_passes > num_pass) num_passes = num_pass;
+        plane = (numbps >> 11) - 1 - missing;
+        if (plane < 0) plane = p_max;
+        if (plane == 0) plane = 1;
+    }
+    if (num_passes > 0) {
+        Mq m;
+        m.init(data + off, len, lane);
+        int type = 2;
+        for (int pi = 0; pi < num_passes; ++pi, ++type) {
+            if (type == 3) {
+                type = 0;
+                --plane;
+            }
+            const uint32_t one_half = (plane >= 0 && plane < 31) ? (1u << plane) + ((1u << plane) >> 1) : 0u;
+            const uint32_t half = (plane >= 1 && plane < 32) ? (1u << plane) >> 1 : 0u;
+            Stripe S;
+            S.pend_next = 0;
+            for (int s = 0; s < nstripes; ++s) {
+                const int rows = min(4, h - 4 * s);
+                S.lo = L.st[s][lane][0] | S.pend_next;
+                S.hi = L.st[s][lane][1];
+                S.v0 = L.val[s][lane][0];
+                S.v1 = L.val[s][lane][1];
+                S.v2 = L.val[s][lane][2];
+                S.v3 = L.val[s][lane][3];
+                S.pend_next = 0;
+                S.pend_prev = 0;
+                uint32_t mlo, mhi;
+                uint64_t todo;
+                if (type == 0) {
+                    // a row that is insignificant with a significant neighbour
+                    rows_mask(rows, 0xFFFF, mlo, mhi);
+                    const uint32_t lo = S.lo & mlo, hi = S.hi & mhi;
+                    bool any = false;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const uint32_t st = ((r < 2 ? lo : hi) >> (16 * (r & 1))) & 0xFFFF;
+                        any = any || (r < rows && !(st & S_SIG) && (st & S_NB));
+                    }
+                    todo = __builtin_amdgcn_ballot_w64(any) & colmask;
+                } else if (type == 1) {
+                    rows_mask(rows, S_MU, mlo, mhi);
+                    todo = __builtin_amdgcn_ballot_w64(((S.lo & mlo) | (S.hi & mhi)) != 0) & colmask;
+                } else {
+                    // a row not yet significant or coded
+                    bool any = false;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const uint32_t st = ((r < 2 ? S.lo : S.hi) >> (16 * (r & 1))) & 0xFFFF;
+                        any = any || (r < rows && !(st & (S_SIG | S_VISIT)));
+                    }
+                    todo = __builtin_amdgcn_ballot_w64(any) & colmask;
+                }
+                while (todo) {
+                    const int x = (int)__builtin_ctzll(todo);
+                    todo &= todo - 1;
+                    uint32_t lo = rdl(S.lo, x), hi = rdl(S.hi, x);
+                    const uint32_t sig0 = (lo | (hi << 1)) & (S_SIG | (S_SIG << 16) | (S_SIG << 1) | (S_SIG << 17));
+                    if (type == 0) {
+                        sigprop_row<0>(m, S, lo, hi, x, w, rows, orient, one_half, lane);
+                        sigprop_row<1>(m, S, lo, hi, x, w, rows, orient, one_half, lane);
+                        sigprop_row<2>(m, S, lo, hi, x, w, rows, orient, one_half, lane);
+                        sigprop_row<3>(m, S, lo, hi, x, w, rows, orient, one_half, lane);
+                        // a sample that became significant makes column x+1 a candidate
+                        const uint32_t sig1 = (lo | (hi << 1)) & (S_SIG | (S_SIG << 16) | (S_SIG << 1) | (S_SIG << 17));
+                        if (x + 1 < w && sig1 != sig0) todo |= 1ull << (x + 1);
+                    } else if (type == 1) {
+                        magref_row<0>(m, S, lo, hi, x, rows, plane, half, lane);
+                        magref_row<1>(m, S, lo, hi, x, rows, plane, half, lane);
+                        magref_row<2>(m, S, lo, hi, x, rows, plane, half, lane);
+                        magref_row<3>(m, S, lo, hi, x, rows, plane, half, lane);
+                        continue;  // refinement leaves the state words unchanged
+                    } else {
+                        int first = 0;
+                        if (rows == 4) {
+                            const uint32_t busy = (S_NB | S_SIG | S_VISIT) | ((S_NB | S_SIG | S_VISIT) << 16);
+                            if (((lo | hi) & busy) == 0) {
+                                if (!m.decode(CX_RL, lane)) continue;  // four zeros
+                                first = (int)m.decode(CX_UNI, lane) << 1;
+                                first |= (int)m.decode(CX_UNI, lane);
+                                switch (first) {
+                                case 0: run_hit<0>(m, S, lo, hi, x, w, one_half, lane); break;
+                                case 1: run_hit<1>(m, S, lo, hi, x, w, one_half, lane); break;
+                                case 2: run_hit<2>(m, S, lo, hi, x, w, one_half, lane); break;
+                                default: run_hit<3>(m, S, lo, hi, x, w, one_half, lane); break;
+                                }
+                                ++first;
+                            }
+                        }
+                        cleanup_row<0>(m, S, lo, hi, x, w, rows, orient, one_half, first, lane);
+                        cleanup_row<1>(m, S, lo, hi, x, w, rows, orient, one_half, first, lane);
+                        cleanup_row<2>(m, S, lo, hi, x, w, rows, orient, one_half, first, lane);
+                        cleanup_row<3>(m, S, lo, hi, x, w, rows, orient, one_half, first, lane);
+                    }
+                    S.lo = wrl(lo, x, S.lo);
+                    S.hi = wrl(hi, x, S.hi);
+                }
+                // write the stripe back; after a cleanup pass "coded" is
+                // cleared and significant samples become refinable
+                uint32_t slo = S.lo, shi = S.hi;
+                if (type == 2) {
+                    const uint32_t vis = S_VISIT | (S_VISIT << 16);
+                    slo = (slo & ~vis) | ((slo & (S_SIG | (S_SIG << 16))) << 3);
+                    shi = (shi & ~vis) | ((shi & (S_SIG | (S_SIG << 16))) << 3);
+                }
+                L.st[s][lane][0] = slo;
+                L.st[s][lane][1] = shi;
+                L.val[s][lane][0] = S.v0;
+                L.val[s][lane][1] = S.v1;
+                L.val[s][lane][2] = S.v2;
+                L.val[s][lane][3] = S.v3;
+                if (s > 0) L.st[s - 1][lane][1] |= S.pend_prev;
+            }
+        }
+    }
+
+    // write the block: lane = column (dec_cb_stripe, dec_t1_comp.cpp:329-367)
+    if ((int)lane < w) {
+        const int sh = 31 - numbps;
+        auto *o = gptr_w<int32_t>((void *)outp);
+        for (int y = 0; y < h; ++y) {
+            const uint32_t v = L.val[y >> 2][lane][y & 3];
+            int32_t r = (int32_t)v;
+            if (kind == BMI_T1_INT)
+                r = (v & 0x80000000u) ? -(int32_t)((v & 0x7FFFFFFFu) >> sh) : (int32_t)(v >> sh);
+            o[(size_t)y * out_pitch + lane] = r;
+        }
+    }
+}
+
+}  // namespace
+}  // namespace bmi
+
+using namespace bmi;
+
+extern "C" int BMI_t1_check_blocks(const BmiT1Block *blocks, int nblocks, int *bad)
+{
+    if (bad) *bad = -1;
+    if (!blocks || nblocks < 0) return bmi_fail(BMI_E_INVALID, "t1_check_blocks: bad args");
+    for (int i = 0; i < nblocks; ++i) {
+        const BmiT1Block &b = blocks[i];
+        int rc = BMI_OK;
+        if (b.width < 1 || b.height < 1 || b.len < 0 || b.num_pass < 0 || b.missing < 0 ||
+            b.orient < 0 || b.orient > 3 || (b.kind != BMI_T1_WORDS && b.kind != BMI_T1_INT) ||
+            !b.out || b.out_pitch < b.width)
+            rc = BMI_E_INVALID;
+        else if (b.width > kMaxSide || b.height > kMaxSide)
+            rc = BMI_E_UNSUPPORTED;
+        else if (b.kind == BMI_T1_INT && (b.numbps < 1 || b.numbps > 31))
+            rc = BMI_E_INVALID;
+        if (rc) {
+            if (bad) *bad = i;
+            return bmi_fail(rc, "t1_check_blocks: block %d rejected", i);
+        }
+    }
+    return BMI_OK;
+}
+
+extern "C" int BMI_t1_decode_blocks(const BmiT1Block *blocks, int nblocks, const void *data,
+                                    StreamId stream)
+{
+    if (!blocks || !data || nblocks < 0 || ((uintptr_t)data & 15))
+        return bmi_fail(BMI_E_INVALID, "t1_decode_blocks: bad args (data must be 16-byte aligned)");
+    if (nblocks == 0) return BMI_OK;
+    const dim3 grid((nblocks + kWavesPerGroup - 1) / kWavesPerGroup), block(64 * kWavesPerGroup);
+    // the check below must see this launch, not an error left by an
+    // earlier, unrelated runtime call of the caller
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(t1_decode_kernel, grid, block, 0, (hipStream_t)stream, blocks, nblocks,
+                       (const uint8_t *)data);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return bmi_fail(BMI_E_HIP, "t1 launch: %s", hipGetErrorString(e));
+    return BMI_OK;
+}

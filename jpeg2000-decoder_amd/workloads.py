@@ -174,3 +174,121 @@ def config4_requests(seed: int, nreq: int, rate: float, lo: int = 256, hi: int =
         reqs.append(dict(arrival=float(t[i]), w=side, h=side, bits=8 if u[i, 2] < 0.5 else 12,
                          filt="53" if u[i, 3] < 0.5 else "97", ncomp=3, levels=5, img=i))
     return reqs
+
+
+# ----------------------------------------------------------- T1 streams --
+# Code streams for the GPU tier-1 decoder, made by the project's own EBCOT
+# encoder (csrc/bmi_t1_encoder.c, built into libbmi_t1enc.so).
+
+_T1ENC = None
+
+
+def _t1enc():
+    global _T1ENC
+    if _T1ENC is None:
+        import ctypes
+        import pathlib
+        path = pathlib.Path(__file__).resolve().parent / "libbmi_t1enc.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} missing: build it with make (or __graft_entry__.build())")
+        L = ctypes.CDLL(str(path))
+        P, i = ctypes.c_void_p, ctypes.c_int
+        L.bmi_t1_encode_block.argtypes = [P, i, i, i, i, i, P, ctypes.c_long,
+                                          ctypes.POINTER(ctypes.c_long), ctypes.POINTER(i),
+                                          ctypes.POINTER(i)]
+        L.bmi_t1_encode_block.restype = i
+        L.bmi_t1_encoder_tables.argtypes = [P, P, P, P]
+        L.bmi_t1_encoder_zc.argtypes = [P, i]
+        L.bmi_t1_encoder_sc.argtypes = [P]
+        _T1ENC = L
+    return _T1ENC
+
+
+def t1_encode_block(q: np.ndarray, mb: int, orient: int, pass_limit: int = 0):
+    """Encode one code-block of quantisation indices q (h, w) with mb
+    magnitude bit-planes -> (bytes, num_passes, missing)."""
+    import ctypes
+    a = np.ascontiguousarray(q, dtype=np.int32)
+    h, w = a.shape
+    cap = 64 + 4 * w * h * max(1, mb) // 3 + 4 * w * h
+    out = np.zeros(cap, dtype=np.uint8)
+    n, npass, miss = ctypes.c_long(), ctypes.c_int(), ctypes.c_int()
+    rc = _t1enc().bmi_t1_encode_block(a.ctypes.data_as(ctypes.c_void_p), w, h, mb, orient,
+                                      pass_limit, out.ctypes.data_as(ctypes.c_void_p), cap,
+                                      ctypes.byref(n), ctypes.byref(npass), ctypes.byref(miss))
+    if rc != 0:
+        raise ValueError(f"t1 encode failed ({rc}): magnitude >= 2^{mb} or buffer too small")
+    return out[:n.value].tobytes(), npass.value, miss.value
+
+
+def band_orient(band_id: int) -> int:
+    """Subband id -> orientation (0 LL, 1 HL, 2 LH, 3 HH)."""
+    return 0 if band_id == 0 else (band_id - 1) % 3 + 1
+
+
+def abs_step_norm(exp: int, mant: int, bit_depth: int = 8) -> float:
+    """mfAbsStepNorm of a 9/7 band from its QCD exponent / mantissa
+    (decoder/cudainterface/src/cudainterface.cpp:712-720)."""
+    step = ((mant & 0x7FF) << 2) | 0x2000
+    n = bit_depth - exp
+    step = step << n if n > 0 else step >> (-n)
+    return float(np.float32(step) * np.float32(2.0 ** -13))
+
+
+def t1_image(w: int, h: int, levels: int, ncomp: int, seed: int = 0, cb: int = 64,
+             filt: str = "97", threads: int = 8, pass_limit: int = 0):
+    """Quantise a synthetic image's subbands and encode every code-block.
+
+    9/7: coefficients as tile_planes (LL ~ N(128,40), detail Laplace(4)*2^-l),
+    deadzone-quantised with the step of a QCD (exponent, mantissa) chosen per
+    band: LL (10, 2047) -> step ~0.5, detail level l (9+l, 0) -> 2^-(l+1);
+    guard bits 1, so numbps = exponent<<11 | mantissa and a band has
+    `exponent` magnitude bit-planes.  5/3: integer coefficients of
+    tile_planes, numbps = 12 (guard bits 1, exponent 12).
+    Returns dict(q=(ncomp,h,w) int32 indices, blocks=[dict], data=bytes,
+    steps=(ncomp, 3L+1) float32)."""
+    from concurrent.futures import ThreadPoolExecutor
+    planes = tile_planes(w, h, levels, filt, ncomp, seed)
+    rects = band_rects(w, h, 0, 0, levels)
+    q = np.zeros((ncomp, h, w), dtype=np.int32)
+    steps = np.ones((ncomp, 3 * levels + 1), dtype=np.float32)
+    qcd = {}
+    for (bid, dl, bx, by, bw, bh) in rects:
+        if filt == "97":
+            exp, mant = (10, 2047) if bid == 0 else (9 + dl, 0)
+            numbps = (exp << 11) | mant
+            mb = exp
+            st = abs_step_norm(exp, mant)
+            qcd[bid] = (numbps, mb)
+            for c in range(ncomp):
+                steps[c, bid] = st
+                v = planes[c, by:by + bh, bx:bx + bw].astype(np.float64) / st
+                q[c, by:by + bh, bx:bx + bw] = (np.sign(v) * np.floor(np.abs(v))).astype(np.int32)
+        else:
+            qcd[bid] = (12, 12)
+            q[:, by:by + bh, bx:bx + bw] = planes[:, by:by + bh, bx:bx + bw]
+    jobs = []
+    for c in range(ncomp):
+        for (bid, dl, bx, by, bw, bh) in rects:
+            for y0 in range(0, bh, cb):
+                for x0 in range(0, bw, cb):
+                    jobs.append((c, bid, bx + x0, by + y0, min(cb, bw - x0), min(cb, bh - y0)))
+
+    def enc(j):
+        c, bid, x, y, bw, bh = j
+        numbps, mb = qcd[bid]
+        data, npass, miss = t1_encode_block(q[c, y:y + bh, x:x + bw], mb, band_orient(bid),
+                                            pass_limit)
+        return data, npass, miss
+
+    with ThreadPoolExecutor(threads) as ex:
+        enc_out = list(ex.map(enc, jobs))
+    blocks, chunks, off = [], [], 0
+    for (c, bid, x, y, bw, bh), (data, npass, miss) in zip(jobs, enc_out):
+        blocks.append(dict(comp=c, band=bid, x=x, y=y, w=bw, h=bh, orient=band_orient(bid),
+                           numbps=qcd[bid][0], missing=miss, num_pass=npass, offset=off,
+                           len=len(data)))
+        chunks.append(data)
+        off += len(data)
+    return dict(q=q, blocks=blocks, data=b"".join(chunks) + b"\xff" * 16, steps=steps,
+                filt=filt, levels=levels, w=w, h=h, ncomp=ncomp)

@@ -92,6 +92,25 @@ void or_ict_float(float *r, float *g, float *b, const float *c0, const float *c1
 
 int or_pixel_bytes(int fmt);
 
+/* ---- EBCOT tier 1 (t1_oracle.c) --------------------------------------- */
+/* decode_code_block (bmi_block_decoder.cpp:589-853) of one w x h code-block
+ * of band orientation `orient` (0 LL, 1 HL, 2 LH, 3 HH): `numbps` as the
+ * reference stores it (9/7: exponent<<11 | mantissa, plus guard bits - 1;
+ * 5/3: exponent + guard bits - 1), `missing` all-zero leading bit-planes,
+ * `num_pass` coding passes in one segment of `len` bytes.  Writes the T1
+ * words (sign bit 31, magnitude below) to out (pitch out_pitch). */
+int or_t1_decode_block(const uint8_t *data, long len, int w, int h, int orient, int numbps,
+                       int missing, int num_pass, int32_t *out, int out_pitch);
+/* dec_cb_stripe's 5/3 conversion (dec_t1_comp.cpp:343-367) */
+void or_t1_words_to_int(const int32_t *words, int w, int h, int pitch, int numbps, int32_t *out,
+                        int out_pitch);
+/* the decoder's MQ table (for the test against ITU-T T.800 Table C.2) and
+ * its zero-coding / sign contexts of a 3x3 neighbourhood (nb9 row-major,
+ * centre 4; bit 0 significant, bit 1 negative); sign returns 2*cx + xor */
+void or_t1_tables(uint16_t *qe, uint8_t *nmps, uint8_t *nlps, uint8_t *sw);
+int or_t1_zc(const uint8_t *nb9, int orient);
+int or_t1_sc(const uint8_t *nb9);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,11 @@ def lib() -> ctypes.CDLL:
         L.or_dc_shift.argtypes = [P, i, P, i, i, i, i, i]
         L.or_ict_float.argtypes = [P, P, P, P, P, P, i, i]
         L.or_pixel_bytes.argtypes = [i]
+        L.or_t1_decode_block.argtypes = [P, ctypes.c_long, i, i, i, i, i, i, P, i]
+        L.or_t1_words_to_int.argtypes = [P, i, i, i, i, P, i]
+        L.or_t1_tables.argtypes = [P, P, P, P]
+        L.or_t1_zc.argtypes = [P, i]
+        L.or_t1_sc.argtypes = [P]
         _LIB = L
     return _LIB
 
@@ -211,3 +216,41 @@ def decode_tile(planes: np.ndarray, *, levels: int, filt: str, out_format: int, 
         comps = recon
         co = -1
     return recon, merge_pack(comps, bit_depth, out_format, co, is_float=(filt == "97"))
+
+
+# ------------------------------------------------------------------- T1 --
+
+def t1_decode_block(data: bytes, w: int, h: int, orient: int, numbps: int, missing: int,
+                    num_pass: int) -> np.ndarray:
+    """decode_code_block: T1 words (int32, sign bit 31) of one code-block."""
+    buf = np.frombuffer(bytes(data) + b"\0", dtype=np.uint8)
+    out = np.zeros((h, w), dtype=np.int32)
+    rc = lib().or_t1_decode_block(_p(buf), len(data), w, h, orient, numbps, missing, num_pass,
+                                  _p(out), w)
+    assert rc == 0, rc
+    return out
+
+
+def t1_words_to_int(words: np.ndarray, numbps: int) -> np.ndarray:
+    a = np.ascontiguousarray(words, dtype=np.int32)
+    h, w = a.shape
+    out = np.empty_like(a)
+    lib().or_t1_words_to_int(_p(a), w, h, w, numbps, _p(out), w)
+    return out
+
+
+def t1_tables():
+    qe = np.zeros(47, np.uint16)
+    nm, nl, sw = (np.zeros(47, np.uint8) for _ in range(3))
+    lib().or_t1_tables(_p(qe), _p(nm), _p(nl), _p(sw))
+    return qe, nm, nl, sw
+
+
+def t1_zc(nb9, orient: int) -> int:
+    a = np.asarray(nb9, dtype=np.uint8)
+    return lib().or_t1_zc(_p(a), orient)
+
+
+def t1_sc(nb9) -> int:
+    a = np.asarray(nb9, dtype=np.uint8)
+    return lib().or_t1_sc(_p(a))
