@@ -229,6 +229,103 @@ def serving_leg(pkg, wl, torch, dev, rates, nreq, budget_gib=64):
     return out
 
 
+def t1_leg(pkg, wl, torch, dev, stream, steps, warmup):
+    """Full tile decode including tier 1: the 4096x2160x3 9/7 image,
+    quantised and split into 64x64 code-blocks encoded by the project's
+    EBCOT encoder; per step the GPU decodes every code-block (one per
+    wavefront) into the coefficient planes, then the fused IDWT + ICT packs
+    24-bit BGR.  Code streams and the block table are resident in HBM."""
+    W, H, L = 4096, 2160, 5
+    t0 = time.perf_counter()
+    img = wl.t1_image(W, H, L, 3, seed=0, filt="97", threads=16)
+    enc_s = time.perf_counter() - t0
+    planes = [torch.zeros((H, W), dtype=torch.int32, device=dev) for _ in range(3)]
+    blocks = []
+    for bd in img["blocks"]:
+        b = pkg.BmiT1Block()
+        b.data_offset, b.len = bd["offset"], bd["len"]
+        b.width, b.height, b.orient = bd["w"], bd["h"], bd["orient"]
+        b.numbps, b.missing, b.num_pass, b.kind = bd["numbps"], bd["missing"], bd["num_pass"], 0
+        b.out = planes[bd["comp"]].data_ptr() + 4 * (bd["y"] * W + bd["x"])
+        b.out_pitch = W
+        blocks.append(b)
+    pkg.t1_check_blocks(blocks)
+    arr = (pkg.BmiT1Block * len(blocks))(*blocks)
+    dblocks = torch.from_numpy(np.frombuffer(bytes(arr), dtype=np.uint8).copy()).to(dev)
+    ddata = torch.from_numpy(np.frombuffer(img["data"], dtype=np.uint8).copy()).to(dev)
+    out = torch.zeros((H, W * 3), dtype=torch.uint8, device=dev)
+    desc = pkg.tile_desc(coef_ptrs=[p.data_ptr() for p in planes], coef_pitch=W, width=W, height=H,
+                         levels=L, method=pkg.W9X7, ncomp=3, coef_kind=pkg.COEF_SIGNMAG,
+                         steps=img["steps"], out_format=pkg.FMT_24BITS_RGB,
+                         out_ptr=out.data_ptr(), out_pitch_px=W)
+    plan = pkg.BatchPlan([desc])
+    sh = stream.cuda_stream
+    nb = len(blocks)
+
+    def step():
+        pkg.t1_decode_blocks(dblocks.data_ptr(), nb, ddata.data_ptr(), sh)
+        plan.execute_graph(sh)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(stream)
+    for _ in range(steps):
+        step()
+    ev[1].record(stream)
+    ev[2].record(stream)
+    for _ in range(steps):
+        pkg.t1_decode_blocks(dblocks.data_ptr(), nb, ddata.data_ptr(), sh)
+    ev[3].record(stream)
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / steps
+    t1_ms = ev[2].elapsed_time(ev[3]) / steps
+    plan.close()
+    ncoef = W * H * 3
+    return {"workload": f"{W}x{H}x3 9/7 L={L}, {nb} code-blocks of 64x64 (own EBCOT encoder, "
+                        "quantised Laplace(4)*2^-l / N(128,40) LL, 3 passes per plane, no bypass)",
+            "code_bytes": len(img["data"]) - 16,
+            "bits_per_coef": round((len(img["data"]) - 16) * 8 / ncoef, 3),
+            "ms_per_image": round(ms, 3), "Mpixel_s": round(W * H / (ms * 1e-3) / 1e6, 1),
+            "t1_ms": round(t1_ms, 3), "t1_Mcoef_s": round(ncoef / (t1_ms * 1e-3) / 1e6, 1),
+            "t1_blocks_s": round(nb / (t1_ms * 1e-3), 0),
+            "encode_s_host": round(enc_s, 2), "_img": img}
+
+
+def t1_cpu_baseline(img, budget_s: float = 8.0):
+    """The oracle's tier-1 decoder (scalar C restatement of
+    decode_code_block) on the same code-blocks, one host thread per core,
+    until about budget_s of wall time."""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle  # checker / CPU baseline only
+    oracle.lib()
+    data = img["data"]
+    blocks = img["blocks"]
+    threads = min(16, os.cpu_count() or 1)
+
+    def one(bd):
+        oracle.t1_decode_block(data[bd["offset"]:bd["offset"] + bd["len"]], bd["w"], bd["h"],
+                               bd["orient"], bd["numbps"], bd["missing"], bd["num_pass"])
+        return bd["w"] * bd["h"]
+
+    t0 = time.perf_counter()
+    ncoef = nblk = 0
+    with ThreadPoolExecutor(threads) as ex:
+        for i in range(0, len(blocks), 256):
+            chunk = blocks[i:i + 256]
+            ncoef += sum(ex.map(one, chunk))
+            nblk += len(chunk)
+            if time.perf_counter() - t0 > budget_s:
+                break
+    el = time.perf_counter() - t0
+    return {"value": round(ncoef / el / 1e6, 2), "unit": "Mcoef/s", "cores": threads,
+            "kind": "port", "blocks_s": round(nblk / el, 1),
+            "sample": f"{nblk} of the {len(blocks)} code-blocks decoded by the C oracle's "
+                      f"decode_code_block on {threads} host threads ({el:.1f} s)"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,6 +343,7 @@ def main() -> int:
     ap.add_argument("--min-waves", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-tile batched leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-t1", action="store_true", help="skip the tier-1 leg")
     ap.add_argument("--serve-rates", default="50,200,800,3200,12800")
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--calib-copy", action="store_true",
@@ -394,11 +492,22 @@ def main() -> int:
         rates = [float(r) for r in args.serve_rates.split(",") if r]
         serving = serving_leg(pkg, wl, torch, dev, rates, args.serve_requests)
 
+    t1 = None
+    if not args.no_t1:
+        log("tier 1: GPU code-block decode + fused IDWT/ICT")
+        t1 = t1_leg(pkg, wl, torch, dev, stream, max(5, args.steps // 5), 2)
+        log(f"t1: {t1['t1_ms']} ms ({t1['t1_Mcoef_s']} Mcoef/s), image {t1['ms_per_image']} ms")
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle)")
         cpu = cpu_baseline(planes, W, H)
         log(f"cpu: {cpu['value']} Mpixel/s")
+        if t1 is not None:
+            cpu["t1"] = t1_cpu_baseline(t1["_img"])
+            log(f"cpu t1: {cpu['t1']['value']} Mcoef/s")
+    if t1 is not None:
+        t1.pop("_img", None)
 
     rec = {
         "metric": METRIC,
@@ -426,6 +535,7 @@ def main() -> int:
         "plain_gpu": plain,
         "batched": batched,
         "serving": serving,
+        "t1": t1,
     }
     plan.close()
     print(json.dumps(rec), flush=True)
