@@ -162,7 +162,7 @@ typedef struct BmiTuning {
     int persistent;      /* 1 = one persistent launch with a device work
                             queue; 0 (default) = one launch per level step */
     int min_waves_per_cu; /* segment heights shrink until each level step
-                             offers this many items per CU */
+                             offers this many items per CU (default 4) */
 } BmiTuning;
 
 typedef struct BmiBatchPlan BmiBatchPlan;

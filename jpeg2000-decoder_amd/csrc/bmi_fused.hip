@@ -766,11 +766,20 @@ __global__ void __launch_bounds__(256) fused_persistent_kernel(const LevelJob *j
 template <int V>
 __global__ void __launch_bounds__(256) fused_step_kernel(const LevelJob *__restrict__ jobs,
                                                          const WorkItem *__restrict__ items,
-                                                         int first, int count)
+                                                         int first, int count, int remap)
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int i = blockIdx.x * (blockDim.x >> 6) + wave;
+    // Workgroups are dispatched to the 8 XCDs round robin (workgroup g on
+    // XCD g % 8).  Give each XCD a contiguous range of items instead, so
+    // the rows and columns that neighbouring items both read (the lifting
+    // halo) are fetched through one XCD's L2.
+    int g = blockIdx.x;
+    if (remap) {
+        const int G = gridDim.x, x = g & 7, j = g >> 3, q = G >> 3, r = G & 7;
+        g = x * q + min(x, r) + j;
+    }
+    const int i = g * (blockDim.x >> 6) + wave;
     if (i >= count) return;
     const int job = __builtin_amdgcn_readfirstlane(items[first + i].job);
     const int local = __builtin_amdgcn_readfirstlane(items[first + i].local);
@@ -913,7 +922,7 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
     P->tuning.waves_per_group = (tuning && tuning->waves_per_group > 0) ? tuning->waves_per_group : 4;
     P->tuning.groups_per_cu = (tuning && tuning->groups_per_cu > 0) ? tuning->groups_per_cu : 8;
     P->tuning.persistent = tuning ? tuning->persistent : 0;
-    P->tuning.min_waves_per_cu = (tuning && tuning->min_waves_per_cu > 0) ? tuning->min_waves_per_cu : 8;
+    P->tuning.min_waves_per_cu = (tuning && tuning->min_waves_per_cu > 0) ? tuning->min_waves_per_cu : 4;
     if (P->tuning.waves_per_group > 4) P->tuning.waves_per_group = 4;
     if (hipGetDevice(&P->device) != hipSuccess) P->device = 0;
     {
@@ -1249,6 +1258,7 @@ static int plan_enqueue(BmiBatchPlan *P, hipStream_t s)
     const int n = (int)P->items.size();
     if (n == 0) return BMI_OK;
     const int wpg = P->tuning.waves_per_group;
+    const int xcd_remap = 1;
     (void)hipGetLastError();  // report this plan's launches, not an earlier error
     if (P->tuning.persistent) {
         const int max_groups = P->num_cus * P->tuning.groups_per_cu;
@@ -1263,7 +1273,7 @@ static int plan_enqueue(BmiBatchPlan *P, hipStream_t s)
 #define BMI_STEP_CASE(V)                                                                        \
     case V:                                                                                    \
         hipLaunchKernelGGL((fused_step_kernel<V>), grid, block, 0, s, P->d_jobs, P->d_items, \
-                           l.first, l.count);                                                  \
+                           l.first, l.count, xcd_remap);                                       \
         break;
                 BMI_FOR_EACH_VARIANT(BMI_STEP_CASE)
 #undef BMI_STEP_CASE
