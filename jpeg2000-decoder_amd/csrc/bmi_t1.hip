@@ -34,7 +34,7 @@ namespace bmi {
 namespace {
 
 constexpr int kMaxSide = 64;
-constexpr int kWavesPerGroup = 2;
+constexpr int kWavesPerGroup = 4;
 
 // T.800 Table C.2 packed: Qe | NMPS << 16 | NLPS << 22 | SWITCH << 28
 #define MQE(q, nm, nl, sw) ((uint32_t)(q) | ((uint32_t)(nm) << 16) | ((uint32_t)(nl) << 22) | ((uint32_t)(sw) << 28))
@@ -111,11 +111,13 @@ __device__ __forceinline__ uint32_t sc_context(uint32_t s)
 
 constexpr int kMaxStripes = kMaxSide / 4;
 
-// Per wave: sample words and state words of the whole block, stripe-major
-// and column-major inside a stripe, so one lane loads one column's four
-// rows with one access.
+// Per wave, LDS holds the state words of the whole block, stripe-major
+// and column-major inside a stripe (one lane loads one column's four rows
+// with one access).  The sample words live in the block's own output
+// rows in global memory: each stripe is loaded one stripe ahead of its
+// use and stored back after it, which keeps LDS at 8 KB per wave and the
+// number of waves per CU high (the decoder is latency-bound).
 struct WaveLds {
-    uint32_t val[kMaxStripes][kMaxSide][4];
     uint32_t st[kMaxStripes][kMaxSide][2];  // 16-bit state of rows 0,1 | 2,3
 };
 
@@ -426,10 +428,18 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup)
     const int nstripes = (h + 3) >> 2;
     const uint64_t colmask = (w == 64) ? ~0ull : ((1ull << w) - 1);
 
-    for (int s = 0; s < nstripes; ++s) {
-        L.val[s][lane][0] = L.val[s][lane][1] = L.val[s][lane][2] = L.val[s][lane][3] = 0;
-        L.st[s][lane][0] = L.st[s][lane][1] = 0;
-    }
+    auto *o = gptr_w<uint32_t>((void *)outp);
+    const bool col_ok = (int)lane < w;
+    for (int s = 0; s < nstripes; ++s) L.st[s][lane][0] = L.st[s][lane][1] = 0;
+    if (col_ok)
+        for (int y = 0; y < h; ++y) o[(size_t)y * out_pitch + lane] = 0;
+    // sample word of row y of this lane's column (0 past the block)
+    auto vload = [&](int y) -> uint32_t {
+        return (col_ok && y < h) ? o[(size_t)y * out_pitch + lane] : 0u;
+    };
+    auto vstore = [&](int y, uint32_t v) {
+        if (col_ok && y < h) o[(size_t)y * out_pitch + lane] = v;
+    };
 
     // decode_code_block :610-649
     int num_passes = 0;
@@ -455,14 +465,22 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup)
             const uint32_t half = (plane >= 1 && plane < 32) ? (1u << plane) >> 1 : 0u;
             Stripe S;
             S.pend_next = 0;
+            uint32_t n0 = vload(0), n1 = vload(1), n2 = vload(2), n3 = vload(3);
             for (int s = 0; s < nstripes; ++s) {
                 const int rows = min(4, h - 4 * s);
                 S.lo = L.st[s][lane][0] | S.pend_next;
                 S.hi = L.st[s][lane][1];
-                S.v0 = L.val[s][lane][0];
-                S.v1 = L.val[s][lane][1];
-                S.v2 = L.val[s][lane][2];
-                S.v3 = L.val[s][lane][3];
+                S.v0 = n0;
+                S.v1 = n1;
+                S.v2 = n2;
+                S.v3 = n3;
+                if (s + 1 < nstripes) {
+                    const int y = 4 * (s + 1);
+                    n0 = vload(y);
+                    n1 = vload(y + 1);
+                    n2 = vload(y + 2);
+                    n3 = vload(y + 3);
+                }
                 S.pend_next = 0;
                 S.pend_prev = 0;
                 uint32_t mlo, mhi;
@@ -545,25 +563,23 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup)
                 }
                 L.st[s][lane][0] = slo;
                 L.st[s][lane][1] = shi;
-                L.val[s][lane][0] = S.v0;
-                L.val[s][lane][1] = S.v1;
-                L.val[s][lane][2] = S.v2;
-                L.val[s][lane][3] = S.v3;
+                vstore(4 * s, S.v0);
+                vstore(4 * s + 1, S.v1);
+                vstore(4 * s + 2, S.v2);
+                vstore(4 * s + 3, S.v3);
                 if (s > 0) L.st[s - 1][lane][1] |= S.pend_prev;
             }
         }
     }
 
-    // write the block: lane = column (dec_cb_stripe, dec_t1_comp.cpp:329-367)
-    if ((int)lane < w) {
+    // 5/3: convert the words in place (dec_cb_stripe, dec_t1_comp.cpp:343-367);
+    // 9/7 words are copied as they are (:329-341)
+    if (kind == BMI_T1_INT && col_ok) {
         const int sh = 31 - numbps;
-        auto *o = gptr_w<int32_t>((void *)outp);
         for (int y = 0; y < h; ++y) {
-            const uint32_t v = L.val[y >> 2][lane][y & 3];
-            int32_t r = (int32_t)v;
-            if (kind == BMI_T1_INT)
-                r = (v & 0x80000000u) ? -(int32_t)((v & 0x7FFFFFFFu) >> sh) : (int32_t)(v >> sh);
-            o[(size_t)y * out_pitch + lane] = r;
+            const uint32_t v = o[(size_t)y * out_pitch + lane];
+            const int32_t r = (v & 0x80000000u) ? -(int32_t)((v & 0x7FFFFFFFu) >> sh) : (int32_t)(v >> sh);
+            o[(size_t)y * out_pitch + lane] = (uint32_t)r;
         }
     }
 }
