@@ -12,7 +12,7 @@
 // state word per sample on a grid with a one-sample border.  The state word
 // carries the significance of the eight neighbours, so a context is formed
 // from a single LDS read:
-//   bits 0..7   significance of NW, N, NE, W, E, SW, S, SE
+//   bits 0..7   significance of N, S, W, E, NW, NE, SW, SE
 //   bit 8       significant        bit 9   negative
 //   bit 10      coded in this plane's significance pass
 //   bit 11      refinable (significant before the current plane)
@@ -57,9 +57,10 @@ __constant__ uint32_t kMqTable[47] = {
     MQE(0x0001, 45, 43, 0),  MQE(0x5601, 46, 46, 0)};
 #undef MQE
 
-// state word bits
-constexpr uint32_t S_NW = 1u << 0, S_N = 1u << 1, S_NE = 1u << 2, S_W = 1u << 3, S_E = 1u << 4,
-                   S_SW = 1u << 5, S_S = 1u << 6, S_SE = 1u << 7;
+// state word bits (the four direct neighbours first, so that the sign
+// context is indexed by bits 0..3 and 12..15)
+constexpr uint32_t S_N = 1u << 0, S_S = 1u << 1, S_W = 1u << 2, S_E = 1u << 3, S_NW = 1u << 4,
+                   S_NE = 1u << 5, S_SW = 1u << 6, S_SE = 1u << 7;
 constexpr uint32_t S_NB = 0xFFu;
 constexpr uint32_t S_SIG = 1u << 8, S_NEG = 1u << 9, S_VISIT = 1u << 10, S_MU = 1u << 11;
 constexpr uint32_t S_NNEG = 1u << 12, S_SNEG = 1u << 13, S_WNEG = 1u << 14, S_ENEG = 1u << 15;
@@ -132,106 +133,136 @@ __device__ __forceinline__ uint32_t wrl(uint32_t s, uint32_t lane, uint32_t v)
     return ((threadIdx.x & 63) == lane) ? s : v;
 }
 
+// Context tables of one code-block, 256 nibbles each, eight per lane in
+// lanes 0..31 of a vector register: zero coding indexed by the eight
+// neighbour bits (for the block's orientation), sign coding indexed by
+// the four direct neighbours' significance and sign bits, holding
+// (context - 9) | xor << 3.
+__device__ __forceinline__ uint32_t build_lut(uint32_t lane, int orient, bool sign)
+{
+    uint32_t word = 0;
+    if (lane < 32)
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t i = lane * 8 + j;
+            uint32_t e;
+            if (sign) {
+                const uint32_t st = (i & 0xF) | ((i & 0xF0) << 8);
+                const uint32_t sc = sc_context(st);
+                e = ((sc & 0xFF) - CX_SC) | ((sc >> 8) << 3);
+            } else {
+                e = zc_context(i, orient);
+            }
+            word |= e << (4 * j);
+        }
+    return word;
+}
+__device__ __forceinline__ uint32_t lut_get(uint32_t lut, uint32_t i)
+{
+    return (rdl(lut, i >> 3) >> (4 * (i & 7))) & 0xF;
+}
+__device__ __forceinline__ uint32_t zc_lut(uint32_t zl, uint32_t s) { return lut_get(zl, s & S_NB); }
+// sign context | xor << 8, as sc_context
+__device__ __forceinline__ uint32_t sc_lut(uint32_t sl, uint32_t s)
+{
+    const uint32_t e = lut_get(sl, (s & 0xF) | ((s >> 8) & 0xF0));
+    return (CX_SC + (e & 7)) | ((e >> 3) << 8);
+}
+
 // MQ decoder (T.800 C.3).  Registers A, C, CT and the byte position are
-// wave-uniform (scalar); the Qe table, the 19 context states and a
-// 2 x 256-byte window of the code stream live one entry per lane in vector
-// registers and are read and written with readlane / writelane.
+// wave-uniform (scalar).  The code stream is read through a 64-bit buffer
+// holding the bytes from the current one on, refilled with scalar loads
+// (the stream is read-only for the kernel).  The Qe table and the 19
+// context states live one entry per lane in vector registers, read with
+// readlane and written with a lane select.
 struct Mq {
     uint32_t A, C;
     int CT;
-    int bp;       // segment index of the current byte
+    int bp;        // segment index of the current byte
     int len;
-    int mis;      // segment start modulo 4
-    uintptr_t abase;  // segment start rounded down to 4 bytes
-    uintptr_t aend;   // first address past the segment
-    int chunk;    // highest 256-byte chunk accessed so far
-    uint32_t win0, win1, win2, win3;  // per lane: dword `lane` of chunk k in slot k & 3
-    uint32_t qtab;        // per lane: Table C.2 entry `lane`
-    uint32_t ctx;         // per lane: state (index << 1 | MPS) of context `lane`
+    const uint8_t *seg;
+    uint64_t buf;  // bytes bp, bp+1, ... (little endian), nbuf of them valid
+    int nbuf;
+    uint32_t qtab;  // per lane: Table C.2 entry `lane`
+    uint32_t ctx;   // per lane: state (index << 1 | MPS) of context `lane`
+    uint32_t zl, sl;  // zero-coding and sign context tables (build_lut)
 
-    __device__ __forceinline__ uint32_t load_chunk(int k, uint32_t lane) const
+    // keep at least 5 bytes from bp on in the buffer; bytes past the
+    // segment are masked by byte_at, the loads stay within the 16 bytes
+    // of slack the caller guarantees after every segment
+    __device__ __forceinline__ void fill()
     {
-        const uintptr_t a = abase + (uintptr_t)k * 256 + 4 * lane;
-        return a < aend ? *gptr<uint32_t>((const void *)a) : 0xFFFFFFFFu;
-    }
-    __device__ __forceinline__ uint32_t byte_at(int i, uint32_t lane)
-    {
-        if (i >= len) return 0xFF;
-        const int j = i + mis;
-        const int k = j >> 8;
-        if (k > chunk) {
-            // entering chunk k (byte accesses move one byte at a time, so
-            // k = chunk + 1): prefetch chunk k + 2 into the slot of chunk
-            // k - 2, which no access can reach any more (accesses are at
-            // the current byte or the one after it)
-            chunk = k;
-            const uint32_t nv = load_chunk(k + 2, lane);
-            switch ((k + 2) & 3) {
-            case 0: win0 = nv; break;
-            case 1: win1 = nv; break;
-            case 2: win2 = nv; break;
-            default: win3 = nv; break;
-            }
+        while (nbuf <= 4) {
+            const uintptr_t a = (uintptr_t)(seg + bp + nbuf);
+            const auto *p = (const __attribute__((address_space(4))) uint32_t *)(a & ~(uintptr_t)3);
+            const int sh = (int)(a & 3);
+            buf |= (uint64_t)(*p >> (8 * sh)) << (8 * nbuf);
+            nbuf += 4 - sh;
         }
-        const uint32_t l = (j >> 2) & 63;
-        uint32_t word;
-        switch (k & 3) {
-        case 0: word = rdl(win0, l); break;
-        case 1: word = rdl(win1, l); break;
-        case 2: word = rdl(win2, l); break;
-        default: word = rdl(win3, l); break;
-        }
-        return (word >> ((j & 3) * 8)) & 0xFF;
     }
-    __device__ __forceinline__ void bytein(uint32_t lane)
+    // byte bp + i (i = 0 or 1); 0xFF past the segment, as the reference's
+    // decoder appends 0xFF 0xFF (bmi_mq_decoder.cpp:195)
+    __device__ __forceinline__ uint32_t byte_at(int i) const
     {
-        if (byte_at(bp, lane) == 0xFF) {
-            const uint32_t b1 = byte_at(bp + 1, lane);
+        return (bp + i >= len) ? 0xFFu : (uint32_t)(buf >> (8 * i)) & 0xFF;
+    }
+    __device__ __forceinline__ void advance()
+    {
+        ++bp;
+        buf >>= 8;
+        --nbuf;
+        fill();
+    }
+    __device__ __forceinline__ void bytein()
+    {
+        if (byte_at(0) == 0xFF) {
+            const uint32_t b1 = byte_at(1);
             if (b1 > 0x8F) {
                 C += 0xFF00;
                 CT = 8;
             } else {
-                ++bp;
+                advance();
                 C += b1 << 9;
                 CT = 7;
             }
         } else {
-            ++bp;
-            C += byte_at(bp, lane) << 8;
+            advance();
+            C += byte_at(0) << 8;
             CT = 8;
         }
     }
-    __device__ __forceinline__ void init(const uint8_t *seg, int n, uint32_t lane)
+    __device__ __forceinline__ void init(const uint8_t *sg, int n, uint32_t lane, int orient)
     {
+        zl = build_lut(lane, orient, false);
+        sl = build_lut(lane, orient, true);
         len = n;
-        const uintptr_t sa = (uintptr_t)seg;
-        mis = (int)(sa & 3);
-        abase = sa & ~(uintptr_t)3;
-        aend = sa + (uintptr_t)n;
-        win0 = load_chunk(0, lane);
-        win1 = load_chunk(1, lane);
-        win2 = load_chunk(2, lane);
-        win3 = 0xFFFFFFFFu;
-        chunk = 0;
+        seg = sg;
+        buf = 0;
+        nbuf = 0;
+        bp = 0;
+        fill();
         qtab = lane < 47 ? kMqTable[lane] : 0u;
         // all contexts index 0 / MPS 0, except zero coding 0 (index 4),
         // run (index 3) and uniform (index 46)
         ctx = lane == 0 ? (4u << 1) : (lane == CX_RL ? (3u << 1) : (lane == CX_UNI ? (46u << 1) : 0u));
-        bp = 0;
-        C = byte_at(0, lane) << 16;
-        bytein(lane);
+        C = byte_at(0) << 16;
+        bytein();
         C <<= 7;
         CT -= 7;
         A = 0x8000;
     }
-    __device__ __forceinline__ void renorm(uint32_t lane)
+    // RENORMD: shift A and C until bit 15 of A is set, reading a byte each
+    // time CT runs out (several shifts at once between reads)
+    __device__ __forceinline__ void renorm()
     {
-        do {
-            if (CT == 0) bytein(lane);
-            A <<= 1;
-            C <<= 1;
-            --CT;
-        } while (!(A & 0x8000));
+        int n = __builtin_clz(A) - 16;
+        while (n > 0) {
+            if (CT == 0) bytein();
+            const int k = n < CT ? n : CT;
+            A <<= k;
+            C <<= k;
+            CT -= k;
+            n -= k;
+        }
     }
     __device__ __forceinline__ uint32_t decode(uint32_t cx, uint32_t lane)
     {
@@ -251,7 +282,7 @@ struct Mq {
             }
             ctx = wrl(ns, cx, ctx);
             A = qe;
-            renorm(lane);
+            renorm();
         } else {
             C -= qe << 16;
             if (!(A & 0x8000)) {
@@ -263,7 +294,7 @@ struct Mq {
                     ns = (((e >> 16) & 63) << 1) | mps;
                 }
                 ctx = wrl(ns, cx, ctx);
-                renorm(lane);
+                renorm();
             } else {
                 d = mps;
             }
@@ -346,8 +377,8 @@ __device__ __forceinline__ void sigprop_row(Mq &m, Stripe &S, uint32_t &lo, uint
     const uint32_t s = row_state<R>(lo, hi);
     if ((s & S_SIG) || !(s & S_NB)) return;
     or_row(lo, hi, R, S_VISIT);
-    if (!m.decode(zc_context(s, orient), lane)) return;
-    const uint32_t sc = sc_context(s);
+    if (!m.decode(zc_lut(m.zl, s), lane)) return;
+    const uint32_t sc = sc_lut(m.sl, s);
     const uint32_t neg = m.decode(sc & 0xFF, lane) ^ (sc >> 8);
     significant<R>(S, lo, hi, x, w, neg);
     vrow<R>(S) = wrl((neg << 31) + one_half, x, vrow<R>(S));
@@ -376,8 +407,8 @@ __device__ __forceinline__ void cleanup_row(Mq &m, Stripe &S, uint32_t &lo, uint
     if (R >= rows || R < first) return;
     const uint32_t s = row_state<R>(lo, hi);
     if (s & (S_SIG | S_VISIT)) return;
-    if (!m.decode(zc_context(s, orient), lane)) return;
-    const uint32_t sc = sc_context(s);
+    if (!m.decode(zc_lut(m.zl, s), lane)) return;
+    const uint32_t sc = sc_lut(m.sl, s);
     const uint32_t neg = m.decode(sc & 0xFF, lane) ^ (sc >> 8);
     significant<R>(S, lo, hi, x, w, neg);
     vrow<R>(S) = wrl((neg << 31) + one_half, x, vrow<R>(S));
@@ -388,7 +419,7 @@ __device__ __forceinline__ void run_hit(Mq &m, Stripe &S, uint32_t &lo, uint32_t
                                         uint32_t one_half, uint32_t lane)
 {
     const uint32_t s = row_state<R>(lo, hi);
-    const uint32_t sc = sc_context(s);
+    const uint32_t sc = sc_lut(m.sl, s);
     const uint32_t neg = m.decode(sc & 0xFF, lane) ^ (sc >> 8);
     significant<R>(S, lo, hi, x, w, neg);
     vrow<R>(S) = wrl((neg << 31) + one_half, x, vrow<R>(S));
@@ -454,7 +485,7 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup)
     }
     if (num_passes > 0) {
         Mq m;
-        m.init(data + off, len, lane);
+        m.init(data + off, len, lane, orient);
         int type = 2;
         for (int pi = 0; pi < num_passes; ++pi, ++type) {
             if (type == 3) {
