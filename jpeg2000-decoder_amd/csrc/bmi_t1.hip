@@ -183,7 +183,7 @@ struct Mq {
     uint64_t buf;  // bytes bp, bp+1, ... (little endian), nbuf of them valid
     int nbuf;
     uint32_t qtab;  // per lane: Table C.2 entry `lane`
-    uint32_t ctx;   // per lane: state (index << 1 | MPS) of context `lane`
+    uint32_t ctx;   // per lane: Table C.2 entry of context `lane`'s index | MPS << 29
     uint32_t zl, sl;  // zero-coding and sign context tables (build_lut)
 
     // keep at least 5 bytes from bp on in the buffer; bytes past the
@@ -243,7 +243,7 @@ struct Mq {
         qtab = lane < 47 ? kMqTable[lane] : 0u;
         // all contexts index 0 / MPS 0, except zero coding 0 (index 4),
         // run (index 3) and uniform (index 46)
-        ctx = lane == 0 ? (4u << 1) : (lane == CX_RL ? (3u << 1) : (lane == CX_UNI ? (46u << 1) : 0u));
+        ctx = kMqTable[lane == 0 ? 4 : (lane == CX_RL ? 3 : (lane == CX_UNI ? 46 : 0))];
         C = byte_at(0) << 16;
         bytein();
         C <<= 7;
@@ -264,23 +264,24 @@ struct Mq {
             n -= k;
         }
     }
+    // One decision in context cx.  Lane cx of `ctx` holds the Table C.2
+    // entry of the context's current index with the MPS in bit 29, so the
+    // common MPS path needs one readlane; the table is read again only
+    // when the index changes.
     __device__ __forceinline__ uint32_t decode(uint32_t cx, uint32_t lane)
     {
-        const uint32_t s = rdl(ctx, cx);
-        const uint32_t idx = s >> 1, mps = s & 1;
-        const uint32_t e = rdl(qtab, idx);
-        const uint32_t qe = e & 0xFFFF;
-        uint32_t d, ns;
+        const uint32_t e = rdl(ctx, cx);
+        const uint32_t qe = e & 0xFFFF, mps = (e >> 29) & 1;
+        uint32_t d;
         A -= qe;
         if ((C >> 16) < qe) {
             if (A < qe) {
                 d = mps;
-                ns = (((e >> 16) & 63) << 1) | mps;
+                ctx = wrl(rdl(qtab, (e >> 16) & 63) | (mps << 29), cx, ctx);
             } else {
                 d = mps ^ 1;
-                ns = (((e >> 22) & 63) << 1) | (mps ^ (e >> 28));
+                ctx = wrl(rdl(qtab, (e >> 22) & 63) | ((mps ^ ((e >> 28) & 1)) << 29), cx, ctx);
             }
-            ctx = wrl(ns, cx, ctx);
             A = qe;
             renorm();
         } else {
@@ -288,12 +289,11 @@ struct Mq {
             if (!(A & 0x8000)) {
                 if (A < qe) {
                     d = mps ^ 1;
-                    ns = (((e >> 22) & 63) << 1) | (mps ^ (e >> 28));
+                    ctx = wrl(rdl(qtab, (e >> 22) & 63) | ((mps ^ ((e >> 28) & 1)) << 29), cx, ctx);
                 } else {
                     d = mps;
-                    ns = (((e >> 16) & 63) << 1) | mps;
+                    ctx = wrl(rdl(qtab, (e >> 16) & 63) | (mps << 29), cx, ctx);
                 }
-                ctx = wrl(ns, cx, ctx);
                 renorm();
             } else {
                 d = mps;
