@@ -194,8 +194,9 @@ int BMI_batch_plan_destroy(BmiBatchPlan *plan);
    (bounded spin) instead of hanging; synchronous. */
 int BMI_batch_plan_error_flag(BmiBatchPlan *plan);
 
-/* Autotuner: build a plan for `tiles` with each candidate tuning, run it
-   once untimed and `reps` times timed with HIP events on `stream`, and
+/* Autotuner: build a plan for `tiles` with each candidate tuning, replay
+   its graph once untimed and `reps` times timed with HIP events on
+   `stream`, and
    return the fastest in *best (its mean time per execute in *best_us).
    candidates == NULL selects the built-in grid (segment rows 16..128 x
    items per CU 4..16 x 2 or 4 waves per workgroup, step mode).  Runs the

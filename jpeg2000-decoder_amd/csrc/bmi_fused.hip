@@ -801,7 +801,7 @@ struct BmiBatchPlan {
     std::vector<WorkItem> items;
     std::vector<int> step_first, step_count;  // level-step mode
     struct Launch {
-        int first, count, variant;
+        int first, count, variant, step;
     };
     std::vector<Launch> launches;  // level-step mode: per step and variant
     std::vector<int> job_tile;                // tile of each job
@@ -1136,7 +1136,7 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
         for (int i = P->step_first[step]; i < (int)P->items.size();) {
             int j = i;
             while (j < (int)P->items.size() && vid(P->items[j]) == vid(P->items[i])) ++j;
-            P->launches.push_back({i, j - i, vid(P->items[i])});
+            P->launches.push_back({i, j - i, vid(P->items[i]), step});
             i = j;
         }
     }
@@ -1252,13 +1252,37 @@ extern "C" int BMI_batch_plan_create(BmiBatchPlan **out_plan, const BmiTileDesc 
     return BMI_OK;
 }
 
-static int plan_enqueue(BmiBatchPlan *P, hipStream_t s)
+static int launch_one(const BmiBatchPlan *P, const BmiBatchPlan::Launch &l, hipStream_t s)
+{
+    const int wpg = P->tuning.waves_per_group;
+    const int xcd_remap = 1;
+    const dim3 grid((l.count + wpg - 1) / wpg), block(64 * wpg);
+    switch (l.variant) {
+#define BMI_STEP_CASE(V)                                                                        \
+    case V:                                                                                    \
+        hipLaunchKernelGGL((fused_step_kernel<V>), grid, block, 0, s, P->d_jobs, P->d_items, \
+                           l.first, l.count, xcd_remap);                                       \
+        break;
+        BMI_FOR_EACH_VARIANT(BMI_STEP_CASE)
+#undef BMI_STEP_CASE
+    default:
+        break;
+    }
+    return BMI_OK;
+}
+
+// Enqueue the plan on stream s.  With `branches` (graph capture only), the
+// launches of one level step -- different kernel variants over disjoint
+// tiles, independent of each other -- are forked onto those streams and
+// joined before the next step, so the captured graph runs them side by
+// side instead of one after another.
+static int plan_enqueue(BmiBatchPlan *P, hipStream_t s, const std::vector<hipStream_t> *branches = nullptr,
+                        std::vector<hipEvent_t> *events = nullptr)
 {
     const int nj = (int)P->jobs.size();
     const int n = (int)P->items.size();
     if (n == 0) return BMI_OK;
     const int wpg = P->tuning.waves_per_group;
-    const int xcd_remap = 1;
     (void)hipGetLastError();  // report this plan's launches, not an earlier error
     if (P->tuning.persistent) {
         const int max_groups = P->num_cus * P->tuning.groups_per_cu;
@@ -1267,19 +1291,38 @@ static int plan_enqueue(BmiBatchPlan *P, hipStream_t s)
         hipLaunchKernelGGL(fused_persistent_kernel, dim3(groups), dim3(64 * wpg), 0, s, P->d_jobs,
                            P->d_items, n, P->d_counters, nj, groups * wpg);
     } else {
-        for (const auto &l : P->launches) {
-            const dim3 grid((l.count + wpg - 1) / wpg), block(64 * wpg);
-            switch (l.variant) {
-#define BMI_STEP_CASE(V)                                                                        \
-    case V:                                                                                    \
-        hipLaunchKernelGGL((fused_step_kernel<V>), grid, block, 0, s, P->d_jobs, P->d_items, \
-                           l.first, l.count, xcd_remap);                                       \
-        break;
-                BMI_FOR_EACH_VARIANT(BMI_STEP_CASE)
-#undef BMI_STEP_CASE
-            default:
-                break;
+        const size_t nl = P->launches.size();
+        for (size_t i = 0; i < nl;) {
+            size_t j = i;
+            while (j < nl && P->launches[j].step == P->launches[i].step) ++j;
+            const int nb = branches ? (int)std::min(branches->size(), j - i) : 0;
+            if (nb <= 1) {
+                for (size_t k = i; k < j; ++k) launch_one(P, P->launches[k], s);
+            } else {
+                // fork: every branch waits for what s has enqueued so far
+                hipEvent_t fork;
+                if (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess)
+                    return bmi_fail(BMI_E_HIP, "fused launch: event");
+                events->push_back(fork);
+                if (hipEventRecord(fork, s) != hipSuccess)
+                    return bmi_fail(BMI_E_HIP, "fused launch: fork");
+                for (int b = 0; b < nb; ++b)
+                    if (hipStreamWaitEvent((*branches)[b], fork, 0) != hipSuccess)
+                        return bmi_fail(BMI_E_HIP, "fused launch: fork wait");
+                for (size_t k = i; k < j; ++k)
+                    launch_one(P, P->launches[k], (*branches)[(k - i) % nb]);
+                // join: s waits for every branch
+                for (int b = 0; b < nb; ++b) {
+                    hipEvent_t join;
+                    if (hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess)
+                        return bmi_fail(BMI_E_HIP, "fused launch: event");
+                    events->push_back(join);
+                    if (hipEventRecord(join, (*branches)[b]) != hipSuccess ||
+                        hipStreamWaitEvent(s, join, 0) != hipSuccess)
+                        return bmi_fail(BMI_E_HIP, "fused launch: join");
+                }
             }
+            i = j;
         }
     }
     hipError_t e = hipGetLastError();
@@ -1324,9 +1367,19 @@ extern "C" int BMI_batch_plan_execute_graph(BmiBatchPlan *P, StreamId stream)
             (void)hipStreamDestroy(cap);
             return bmi_fail(BMI_E_HIP, "graph: begin capture");
         }
-        int rc = plan_enqueue(P, cap);
+        // independent launches of a level step become parallel branches
+        std::vector<hipStream_t> branches;
+        std::vector<hipEvent_t> events;
+        for (int b = 0; b < 4; ++b) {
+            hipStream_t bs;
+            if (hipStreamCreateWithFlags(&bs, hipStreamNonBlocking) != hipSuccess) break;
+            branches.push_back(bs);
+        }
+        int rc = plan_enqueue(P, cap, &branches, &events);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(cap, &g);
+        for (hipStream_t bs : branches) (void)hipStreamDestroy(bs);
+        for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
         (void)hipStreamDestroy(cap);
         if (rc || e != hipSuccess || !g) return bmi_fail(BMI_E_HIP, "graph: capture failed");
         if (hipGraphInstantiate(&P->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
@@ -1432,10 +1485,11 @@ extern "C" int BMI_autotune(const BmiTileDesc *tiles, int ntiles, const BmiTunin
         BmiBatchPlan *P = nullptr;
         rc = BMI_batch_plan_create(&P, tiles, ntiles, &cand[i]);
         if (rc) break;
-        // warm-up (also first-touch of the scratch), then `reps` timed runs
-        rc = plan_enqueue(P, s);
+        // warm-up (captures the plan's graph and first-touches the
+        // scratch), then `reps` timed graph replays, the way plans run
+        rc = BMI_batch_plan_execute_graph(P, stream);
         if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = BMI_E_HIP;
-        for (int r = 0; r < reps && !rc; ++r) rc = plan_enqueue(P, s);
+        for (int r = 0; r < reps && !rc; ++r) rc = BMI_batch_plan_execute_graph(P, stream);
         if (!rc && hipEventRecord(e1, s) != hipSuccess) rc = BMI_E_HIP;
         float ms = 0.0f;
         if (!rc && (hipEventSynchronize(e1) != hipSuccess ||
