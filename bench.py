@@ -344,7 +344,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-tile batched leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-t1", action="store_true", help="skip the tier-1 leg")
-    ap.add_argument("--serve-rates", default="50,200,800,3200,12800")
+    ap.add_argument("--serve-rates", default="50,400,3200,12800,51200")
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 256 MiB device copy, the reference for scaling HBM counters")

@@ -1125,20 +1125,31 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                     J.n_items = J.nstrips * J.nsegs;
                 }
             }
+            // the step's items grouped by kernel variant (jobs stable-sorted
+            // by variant, tile order kept), one launch per variant
+            std::vector<std::pair<int, int>> order;  // (variant, job)
             for (int j = step_job0; j < (int)P->jobs.size(); ++j)
-                for (int li = 0; li < P->jobs[j].n_items; ++li) P->items.push_back(WorkItem{j, li});
+                order.emplace_back(variant_of(P->jobs[j]), j);
+            std::stable_sort(order.begin(), order.end(),
+                             [](const std::pair<int, int> &a, const std::pair<int, int> &b) {
+                                 return a.first < b.first;
+                             });
+            long step_items = 0;
+            for (const auto &o : order) step_items += P->jobs[o.second].n_items;
+            P->items.reserve(P->items.size() + step_items);
+            for (size_t i = 0; i < order.size();) {
+                size_t k = i;
+                const int first = (int)P->items.size();
+                for (; k < order.size() && order[k].first == order[i].first; ++k) {
+                    const int j = order[k].second;
+                    const int n = P->jobs[j].n_items;
+                    for (int li = 0; li < n; ++li) P->items.push_back(WorkItem{j, li});
+                }
+                P->launches.push_back({first, (int)P->items.size() - first, order[i].first, step});
+                i = k;
+            }
         }
         P->step_count[step] = (int)P->items.size() - P->step_first[step];
-        // group the step's items by kernel variant (stable: tile order kept)
-        auto vid = [&](const WorkItem &w) { return variant_of(P->jobs[w.job]); };
-        std::stable_sort(P->items.begin() + P->step_first[step], P->items.end(),
-                         [&](const WorkItem &x, const WorkItem &y) { return vid(x) < vid(y); });
-        for (int i = P->step_first[step]; i < (int)P->items.size();) {
-            int j = i;
-            while (j < (int)P->items.size() && vid(P->items[j]) == vid(P->items[i])) ++j;
-            P->launches.push_back({i, j - i, vid(P->items[i]), step});
-            i = j;
-        }
     }
 
     const size_t jb = sizeof(LevelJob) * P->jobs.size();
