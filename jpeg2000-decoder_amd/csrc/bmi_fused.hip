@@ -233,16 +233,27 @@ struct Emitter {
         return lay == PL_BGR8 ? 3 : lay == PL_BGRA8 ? 4 : lay == PL_BGR16 ? 6 : 8;
     }
 
+    // 32-bit byte offsets from the uniform bases (destinations are limited
+    // to 2^31 bytes by validate_tile), so stores take a scalar base
+    template <typename U>
+    __device__ static inline __attribute__((address_space(1))) U *at(__attribute__((address_space(1))) U *base,
+                                                                      uint32_t byte_off)
+    {
+        return (__attribute__((address_space(1))) U *)((__attribute__((address_space(1))) char *)base + byte_off);
+    }
+
     __device__ inline void put(int row, int col, const T *v) const
     {
         if constexpr (OUTK == K_PLANE) {
-            const size_t idx = (size_t)(row - b0) * pitch + (col - a0);
+            const uint32_t off = ((uint32_t)(row - b0) * (uint32_t)pitch + (uint32_t)(col - a0)) * 4u;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) dst[c][idx] = v[c];
+            for (int c = 0; c < NC; ++c) *at(dst[c], off) = v[c];
         } else {
             int R, G, B, A;
             colour(v, R, G, B, A);
-            store(out + ((size_t)(oy + row) * pitch + (ox + col)) * layout_bytes(), R, G, B, A);
+            store(at(out, ((uint32_t)(oy + row) * (uint32_t)pitch + (uint32_t)(ox + col)) *
+                              (uint32_t)layout_bytes()),
+                  R, G, B, A);
         }
     }
 
@@ -251,14 +262,14 @@ struct Emitter {
     {
         if (pair_fast && e_ok && o_ok) {
             if constexpr (OUTK == K_PLANE) {
-                const size_t idx = (size_t)(row - b0) * pitch + (ce - a0);
+                const uint32_t off = ((uint32_t)(row - b0) * (uint32_t)pitch + (uint32_t)(ce - a0)) * 4u;
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
                     typedef T v2 __attribute__((ext_vector_type(2)));
                     v2 pairv;
                     pairv.x = ve[c];
                     pairv.y = vo[c];
-                    *(__attribute__((address_space(1))) v2 *)(dst[c] + idx) = pairv;
+                    *(__attribute__((address_space(1))) v2 *)at(dst[c], off) = pairv;
                 }
                 return;
             } else if constexpr (PL == PL_BGR8) {
@@ -267,7 +278,7 @@ struct Emitter {
                 int R0, G0, B0, A0, R1, G1, B1, A1;
                 colour(ve, R0, G0, B0, A0);
                 colour(vo, R1, G1, B1, A1);
-                const size_t off = ((size_t)(oy + row) * pitch + (ox + ce)) * 3;
+                const uint32_t off = ((uint32_t)(oy + row) * (uint32_t)pitch + (uint32_t)(ox + ce)) * 3u;
                 const bool mid = (off & 2) != 0;
                 const uint32_t lo4 = (uint32_t)(B0 & 0xff) | ((uint32_t)(G0 & 0xff) << 8) |
                                      ((uint32_t)(R0 & 0xff) << 16) | ((uint32_t)(B1 & 0xff) << 24);
@@ -275,10 +286,10 @@ struct Emitter {
                                      ((uint32_t)(G1 & 0xff) << 16) | ((uint32_t)(R1 & 0xff) << 24);
                 const uint32_t sh_lo = (uint32_t)(B0 & 0xff) | ((uint32_t)(G0 & 0xff) << 8);
                 const uint32_t sh_hi = (uint32_t)(G1 & 0xff) | ((uint32_t)(R1 & 0xff) << 8);
-                const size_t dw_off = mid ? off + 2 : off;
-                const size_t sh_off = mid ? off : off + 4;
-                *(__attribute__((address_space(1))) uint32_t *)(out + dw_off) = mid ? hi4 : lo4;
-                *(__attribute__((address_space(1))) uint16_t *)(out + sh_off) = (uint16_t)(mid ? sh_lo : sh_hi);
+                const uint32_t dw_off = mid ? off + 2 : off;
+                const uint32_t sh_off = mid ? off : off + 4;
+                *(__attribute__((address_space(1))) uint32_t *)at(out, dw_off) = mid ? hi4 : lo4;
+                *(__attribute__((address_space(1))) uint16_t *)at(out, sh_off) = (uint16_t)(mid ? sh_lo : sh_hi);
                 return;
             }
         }
@@ -380,14 +391,23 @@ __device__ __attribute__((always_inline)) inline void process_item(
     };
     auto fetch = [&](int k, uint32_t (&r)[NC][4]) {
         const int rre = reflect_row(2 * k), rro = reflect_row(2 * k + 1);
-        const size_t rt = (size_t)((rre >> 1) - rl0);
-        const size_t q = (size_t)((rro >> 1) - rh0 + lh);
+        // 32-bit element offsets from the (uniform) plane bases: the loads
+        // take a scalar base and a 32-bit vector offset (planes are
+        // limited to 2^31 bytes by validate_tile)
+        const uint32_t rt = (uint32_t)((rre >> 1) - rl0);
+        const uint32_t q = (uint32_t)((rro >> 1) - rh0 + lh);
+        const uint32_t llr = rt * (uint32_t)ll_pitch, dtr = rt * (uint32_t)det_pitch;
+        const uint32_t dbr = q * (uint32_t)det_pitch;
+        auto ld = [](const __attribute__((address_space(1))) uint32_t *base, uint32_t elem) {
+            return *(const __attribute__((address_space(1))) uint32_t *)(
+                (const __attribute__((address_space(1))) char *)base + (elem << 2));
+        };
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-            r[c][0] = llp[c][rt * ll_pitch + lo_idx];
-            r[c][1] = detp[c][rt * det_pitch + hi_off];
-            r[c][2] = detp[c][q * det_pitch + lo_idx];
-            r[c][3] = detp[c][q * det_pitch + hi_off];
+            r[c][0] = ld(llp[c], llr + (uint32_t)lo_idx);
+            r[c][1] = ld(detp[c], dtr + (uint32_t)hi_off);
+            r[c][2] = ld(detp[c], dbr + (uint32_t)lo_idx);
+            r[c][3] = ld(detp[c], dbr + (uint32_t)hi_off);
         }
     };
 
@@ -861,6 +881,10 @@ int validate_tile(const BmiTileDesc &t)
         return BMI_E_INVALID;
     if (t.bit_depth < 1 || t.bit_depth > 16) return BMI_E_INVALID;
     if (t.coef_pitch < t.width) return BMI_E_INVALID;
+    // planes addressed with 32-bit offsets: at most 2^31 bytes each
+    if ((long long)t.coef_pitch * t.height * 4 > (1ll << 31)) return BMI_E_UNSUPPORTED;
+    if (t.out_format == IMAGE_FOTMAT_NONE && (long long)t.planes_pitch * t.height * 4 > (1ll << 31))
+        return BMI_E_UNSUPPORTED;
     const int ok = out_kind_for(t);
     if (ok < 0) return BMI_E_UNSUPPORTED;
     if (ok == OUT_RGB && t.ncomp < 3) return BMI_E_UNSUPPORTED;
@@ -871,6 +895,8 @@ int validate_tile(const BmiTileDesc &t)
             if (!t.planes_out[c]) return BMI_E_INVALID;
     } else if (!t.out || t.out_pitch_px < 1 || t.out_x < 0 || t.out_y < 0) {
         return BMI_E_INVALID;
+    } else if ((long long)t.out_pitch_px * (t.out_y + t.height) * 8 > (1ll << 31)) {
+        return BMI_E_UNSUPPORTED;  // packed output addressed with 32-bit offsets
     }
     if (t.r53_mode != BMI_R53_REFERENCE && t.r53_mode != BMI_R53_TEXTBOOK) return BMI_E_INVALID;
     return BMI_OK;

@@ -492,6 +492,9 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup)
                 type = 0;
                 --plane;
             }
+            // passes below bit-plane 0 are not decoded (a negative shift
+            // count, undefined in the reference)
+            if (plane < 0) break;
             const uint32_t one_half = (plane >= 0 && plane < 31) ? (1u << plane) + ((1u << plane) >> 1) : 0u;
             const uint32_t half = (plane >= 1 && plane < 32) ? (1u << plane) >> 1 : 0u;
             Stripe S;

@@ -225,6 +225,10 @@ int or_t1_decode_block(const uint8_t *data, long len, int w, int h, int orient, 
             type = 0;
             --plane;
         }
+        /* a pass below bit-plane 0 (more passes than the band has planes)
+           would shift by a negative count, undefined in the reference:
+           such passes are not decoded */
+        if (plane < 0) break;
         const uint32_t one_half = (plane >= 0 && plane < 31)
                                       ? ((uint32_t)1 << plane) + (((uint32_t)1 << plane) >> 1)
                                       : 0;

@@ -176,3 +176,55 @@ def test_image_t1_then_idwt_ict(pkg, workloads, orc, filt):
                              coef_kind=orc.COEF_SIGNMAG)
     got = out.cpu().numpy()
     assert np.array_equal(got, exp), diff_report(got, exp)
+
+
+def test_arbitrary_bytes_decode_like_the_oracle(pkg, orc):
+    """Code streams of random bytes (with runs of 0xFF and marker-like byte
+    pairs) and random pass counts: the GPU decoder reads only inside the
+    segment, always terminates, and produces the oracle's words."""
+    import torch
+    rng = np.random.default_rng(91)
+    cases = []
+    for i in range(96):
+        h, w = int(rng.integers(1, 65)), int(rng.integers(1, 65))
+        n = int(rng.integers(0, 600))
+        data = rng.integers(0, 256, n).astype(np.uint8)
+        if n and i % 3 == 0:
+            data[rng.random(n) < 0.2] = 0xFF
+        if n > 4 and i % 5 == 1:
+            data[n // 2] = 0xFF
+            data[n // 2 + 1] = 0x90 + (i % 0x60)
+        kind = i % 2
+        mb = int(rng.integers(1, 16))
+        numbps = ((mb << 11) | int(rng.integers(0, 2048))) if kind == 0 else mb
+        missing = int(rng.integers(0, max(1, mb)))
+        # up to two passes more than the band's bit-planes hold
+        num_pass = int(rng.integers(0, 3 * (mb - missing) + 1))
+        cases.append(dict(h=h, w=w, data=data.tobytes(), kind=kind, numbps=numbps, missing=missing,
+                          num_pass=num_pass, orient=i % 4))
+    pitch = 64 * 16
+    out = torch.full((64 * ((len(cases) + 15) // 16), pitch), 5, dtype=torch.int32, device="cuda")
+    blocks, chunks, off = [], [], 0
+    for i, c in enumerate(cases):
+        b = pkg.BmiT1Block()
+        b.data_offset, b.len = off, len(c["data"])
+        b.width, b.height, b.orient = c["w"], c["h"], c["orient"]
+        b.numbps, b.missing, b.num_pass, b.kind = c["numbps"], c["missing"], c["num_pass"], c["kind"]
+        b.out = out.data_ptr() + 4 * (64 * (i // 16) * pitch + 64 * (i % 16))
+        b.out_pitch = pitch
+        blocks.append(b)
+        chunks.append(c["data"])
+        off += len(c["data"])
+    pkg.t1_check_blocks(blocks)
+    dblocks = upload_blocks(pkg, blocks)
+    ddata = upload_data(b"".join(chunks))
+    pkg.t1_decode_blocks(dblocks.data_ptr(), len(blocks), ddata.data_ptr(), 0)
+    got_all = out.cpu().numpy()
+    for i, c in enumerate(cases):
+        y0, x0 = 64 * (i // 16), 64 * (i % 16)
+        got = got_all[y0:y0 + c["h"], x0:x0 + c["w"]]
+        exp = orc.t1_decode_block(c["data"], c["w"], c["h"], c["orient"], c["numbps"],
+                                  c["missing"], c["num_pass"])
+        if c["kind"] == 1:
+            exp = orc.t1_words_to_int(exp, c["numbps"])
+        assert np.array_equal(got, exp), (i, c["h"], c["w"], c["num_pass"], diff_report(got, exp))
