@@ -158,6 +158,20 @@ class BmiT1Block(ctypes.Structure):
     ]
 
 
+# BMI_codestream_open flags (include/bmi_codestream.h)
+CS_REFERENCE = 0
+CS_GUARD_T800 = 1
+
+
+class BmiCodestreamInfo(ctypes.Structure):
+    """struct BmiCodestreamInfo (include/bmi_codestream.h)."""
+
+    _fields_ = [(n, ctypes.c_int) for n in (
+        "width", "height", "x0", "y0", "tile_w", "tile_h", "tile_x0", "tile_y0", "ntiles_x",
+        "ntiles_y", "ncomp", "bit_depth", "levels", "reversible", "mct", "layers", "progression",
+        "cb_w", "cb_h", "guard_bits", "colour_space", "total_blocks")]
+
+
 class GpuInfo(ctypes.Structure):
     _fields_ = [
         ("active", ctypes.c_int),
@@ -243,6 +257,18 @@ def lib() -> ctypes.CDLL:
     L.BMI_t1_check_blocks.restype = i
     L.BMI_plain_decode.argtypes = [ctypes.POINTER(BmiTileDesc), vp, ctypes.c_size_t, vp]
     L.BMI_plain_decode.restype = i
+    L.BMI_codestream_open.argtypes = [vp, ctypes.c_size_t, i, ctypes.POINTER(vp),
+                                      ctypes.POINTER(BmiCodestreamInfo)]
+    L.BMI_codestream_open.restype = i
+    ip = ctypes.POINTER(i)
+    L.BMI_codestream_tile_info.argtypes = [vp, i, ip, ip, ip, ip, ip, ctypes.POINTER(ctypes.c_size_t)]
+    L.BMI_codestream_tile_info.restype = i
+    L.BMI_codestream_tile_blocks.argtypes = [vp, i, ctypes.POINTER(vp), i, ctypes.c_ulonglong,
+                                             ctypes.POINTER(BmiT1Block), vp, ctypes.c_size_t,
+                                             ctypes.POINTER(BmiTileDesc)]
+    L.BMI_codestream_tile_blocks.restype = i
+    L.BMI_codestream_close.argtypes = [vp]
+    L.BMI_codestream_close.restype = i
     _LIB = L
     return L
 
@@ -445,3 +471,201 @@ def t1_decode_blocks(blocks_dev_ptr: int, nblocks: int, data_dev_ptr: int, strea
     """BMI_t1_decode_blocks on device block table and code-stream buffer."""
     check(lib().BMI_t1_decode_blocks(blocks_dev_ptr, nblocks, data_dev_ptr, stream or None),
           "BMI_t1_decode_blocks")
+
+
+# ------------------------------------------------------------ codestream --
+
+class Codestream:
+    """BMI_codestream_* wrapper: host-side parse and tier-2 decode of a JPEG
+    2000 codestream (.j2k) or JP2 file, feeding the GPU tier-1 decoder and
+    the batched IDWT / colour plan (the reference's BMI_dec_info + dec_t2,
+    decoder/codec/src/dec_jp2.cpp:420-560, dec_tsk.cpp:107-185)."""
+
+    def __init__(self, data: bytes, flags: int = CS_REFERENCE):
+        L = lib()
+        self._buf = bytes(data)
+        self.handle = ctypes.c_void_p()
+        self.info = BmiCodestreamInfo()
+        check(L.BMI_codestream_open(self._buf, len(self._buf), flags, ctypes.byref(self.handle),
+                                    ctypes.byref(self.info)), "BMI_codestream_open")
+
+    @property
+    def ntiles(self) -> int:
+        return self.info.ntiles_x * self.info.ntiles_y
+
+    def tile_info(self, t: int):
+        """(x0, y0, w, h, nblocks, seg_bytes) of tile t."""
+        v = [ctypes.c_int() for _ in range(5)]
+        sb = ctypes.c_size_t()
+        check(lib().BMI_codestream_tile_info(self.handle, t, *[ctypes.byref(x) for x in v],
+                                             ctypes.byref(sb)), "BMI_codestream_tile_info")
+        return tuple(x.value for x in v) + (sb.value,)
+
+    def tile_blocks(self, t: int, plane_ptrs, plane_pitch: int, seg_base: int = 0, segs=None):
+        """Block table, segments and tile descriptor of tile t.  `segs` may
+        be a writable uint8 numpy view of at least seg_bytes + 16 bytes;
+        otherwise one is allocated.  Returns (BmiT1Block array, segs, desc)."""
+        import numpy as np
+        _, _, _, _, nb, sb = self.tile_info(t)
+        blocks = (BmiT1Block * max(nb, 1))()
+        if segs is None:
+            segs = np.zeros(sb + 16, dtype=np.uint8)
+        planes = (ctypes.c_void_p * MAX_COMPONENT)(*[p or None for p in plane_ptrs])
+        desc = BmiTileDesc()
+        check(lib().BMI_codestream_tile_blocks(self.handle, t, planes, plane_pitch, seg_base, blocks,
+                                               segs.ctypes.data_as(ctypes.c_void_p), segs.nbytes,
+                                               ctypes.byref(desc)), "BMI_codestream_tile_blocks")
+        return blocks, segs, desc
+
+    def close(self) -> None:
+        if self.handle:
+            lib().BMI_codestream_close(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_format(info: BmiCodestreamInfo):
+    """Packed output format and check_out for a codestream's components:
+    RCT / ICT to BGR when the stream signals a component transform, the
+    merge formats otherwise (a single component as grey in every slot)."""
+    wide = info.bit_depth > 8
+    if info.ncomp >= 3 and info.mct and info.ncomp == 3:
+        return (FMT_48BITS_RGB if wide else FMT_24BITS_RGB), -1
+    if info.ncomp == 4:
+        return (FMT_64BITS_4COMPONENTS if wide else FMT_32BITS_4COMPONENTS), -1
+    if info.ncomp == 1:
+        return (FMT_48BITS_3COMPONENTS if wide else FMT_24BITS_3COMPONENTS), 0
+    return (FMT_48BITS_3COMPONENTS if wide else FMT_24BITS_3COMPONENTS), -1
+
+
+def _check_block_table(blocks, nb: int, plane_ptrs, pitch: int, rows: int, seg_base: int,
+                       seg_bytes: int) -> None:
+    """Host-side bounds check of a tile's block table before it reaches
+    the GPU: every block's output rectangle lies inside its tile plane and
+    every segment inside the tile's share of the segment buffer."""
+    if nb:
+        t1_check_blocks([blocks[i] for i in range(nb)])
+    for i in range(nb):
+        b = blocks[i]
+        c = next((k for k, p in enumerate(plane_ptrs) if p <= b.out < p + 4 * pitch * rows), -1)
+        if c < 0:
+            raise BmiError(f"block {i}: output outside the tile planes")
+        y, x = divmod((b.out - plane_ptrs[c]) // 4, pitch)
+        if x + b.width > pitch or y + b.height > rows or b.out_pitch != pitch:
+            raise BmiError(f"block {i}: output rectangle outside the tile plane")
+        if b.data_offset < seg_base or b.data_offset + b.len > seg_base + seg_bytes:
+            raise BmiError(f"block {i}: segment outside the tile's data")
+
+
+class CodestreamDecoder:
+    """Decode a whole codestream on the GPU: host tier-2, one tier-1 launch
+    over every code-block of every tile, then one batched IDWT / colour plan
+    over all tiles writing into one output image.  Device buffers are torch
+    tensors kept for the life of the object, so run() can be repeated
+    (e.g. timed) on the resident inputs."""
+
+    def __init__(self, data: bytes, *, flags: int = CS_REFERENCE, out_format=None, check_out=None,
+                 r53_mode: int = R53_REFERENCE, device: str = "cuda"):
+        import numpy as np
+        import torch
+        self.cs = cs = Codestream(data, flags)
+        info = cs.info
+        fmt, co = default_format(info)
+        self.out_format = fmt if out_format is None else out_format
+        self.check_out = co if check_out is None else check_out
+        W, H = info.width, info.height
+        self.planes, descs, tables, seg_parts = [], [], [], []
+        self.out_planes = []
+        if self.out_format == FMT_NONE:
+            for c in range(info.ncomp):
+                self.out_planes.append(torch.zeros((H, W), dtype=torch.int32, device=device))
+            self.out = None
+        else:
+            self.out = torch.zeros((H, W * PIXEL_BYTES[self.out_format]), dtype=torch.uint8,
+                                   device=device)
+        seg_base = 0
+        for t in range(cs.ntiles):
+            x0, y0, w, h, nb, sb = cs.tile_info(t)
+            pitch = (w + 63) // 64 * 64
+            pl = torch.zeros((info.ncomp, h, pitch), dtype=torch.int32, device=device)
+            self.planes.append(pl)
+            ptrs = [pl[c].data_ptr() for c in range(info.ncomp)]
+            blocks, segs, desc = cs.tile_blocks(t, ptrs, pitch, seg_base)
+            desc.out_format = self.out_format
+            desc.check_out = self.check_out
+            desc.r53_mode = r53_mode
+            ox, oy = x0 - info.x0, y0 - info.y0
+            if self.out is not None:
+                desc.out = self.out.data_ptr()
+                desc.out_pitch_px = W
+                desc.out_x, desc.out_y = ox, oy
+            else:
+                for c in range(info.ncomp):
+                    desc.planes_out[c] = self.out_planes[c][oy:, ox:].data_ptr()
+                desc.planes_pitch = W
+            descs.append(desc)
+            _check_block_table(blocks, nb, ptrs, pitch, h, seg_base, sb)
+            tables.append(bytes(blocks)[:nb * ctypes.sizeof(BmiT1Block)])
+            seg_parts.append(segs[:sb].tobytes())
+            seg_base += sb
+        self.nblocks = info.total_blocks
+        seg_all = b"".join(seg_parts) + b"\xff" * 16
+        self.seg_bytes = seg_base
+        self.d_segs = torch.from_numpy(np.frombuffer(seg_all, np.uint8).copy()).to(device)
+        tab = b"".join(tables)
+        self.d_blocks = torch.from_numpy(np.frombuffer(tab or b"\0", np.uint8).copy()).to(device)
+        self.descs = descs
+        # tiles without decomposition levels have no synthesis: the plain
+        # sequence (colour / merge of the coefficient planes) serves them;
+        # for 9/7 the dequantisation happens in the synthesis row pass, so
+        # such tiles are refused (as the reference cannot decode them)
+        fused = [d for d in descs if d.levels > 0]
+        self.plain = [(t, d) for t, d in enumerate(descs) if d.levels == 0]
+        if any(d.method == W9X7 for _, d in self.plain):
+            raise BmiError("9/7 tile without decomposition levels: unsupported (-5)")
+        self.plan = BatchPlan(fused) if fused else None
+        self.temp = None
+        if self.plain:
+            n = max(d.width * d.height for _, d in self.plain) + 64
+            self.temp = torch.zeros(n, dtype=torch.int32, device=device)
+
+    def run_t1(self, stream: int = 0) -> None:
+        if self.nblocks:
+            t1_decode_blocks(self.d_blocks.data_ptr(), self.nblocks, self.d_segs.data_ptr(), stream)
+
+    def run(self, stream: int = 0) -> None:
+        """Tier 1 then IDWT / colour, asynchronous on `stream`."""
+        self.run_t1(stream)
+        if self.plan is not None:
+            self.plan.execute(stream)
+        for t, d in self.plain:
+            plain_decode(d, self.temp.data_ptr(), self.temp.numel() * 4, stream)
+            if self.out is None:
+                # the plain sequence leaves planes in the coefficient buffers
+                ox, oy = d.x0 - self.cs.info.x0, d.y0 - self.cs.info.y0
+                for c in range(d.ncomp):
+                    self.out_planes[c][oy:oy + d.height, ox:ox + d.width].copy_(
+                        self.planes[t][c, :d.height, :d.width])
+
+    def close(self) -> None:
+        if self.plan is not None:
+            self.plan.close()
+        self.cs.close()
+
+
+def decode_codestream(data: bytes, **kw):
+    """Decode a .j2k / .jp2 on the GPU: returns the packed image (uint8
+    tensor, rows of width * pixel bytes) or, with out_format=FMT_NONE, the
+    list of component planes (int32 bits; float32 for 9/7)."""
+    import torch
+    dec = CodestreamDecoder(data, **kw)
+    dec.run(0)
+    torch.cuda.synchronize()
+    out = dec.out if dec.out is not None else dec.out_planes
+    dec.close()
+    return out

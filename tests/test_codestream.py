@@ -1,0 +1,444 @@
+"""Codestream front end (tier 2) on the CPU.
+
+Pins of the oracle (oracle/t2_oracle.py):
+  - the coding-pass codeword, the stuffed bit reader and tag trees against
+    independent encoders written here from ITU-T T.800 B.10 (tag trees:
+    brute force over small grids and values);
+  - whole streams written by an independent encoder (OpenJPEG, fixtures
+    under tests/golden/j2k): lossless streams decode back to the source
+    image exactly through the oracle's T2 + T1 + T.800 5/3 IDWT + RCT;
+    lossy 9/7 streams within 2 levels of OpenJPEG's own decode (T.800
+    bit-plane placement).
+Parity of the product's C++ parser (libbmi_j2k.so, host code, no GPU
+needed): block tables, segments, steps and tile geometry equal the
+oracle's on every fixture, and on the fixtures rewritten with SOP / EPH
+markers, several tile-parts per tile and truncated tiles; malformed and
+unsupported streams are refused with the documented error codes."""
+import ctypes
+import io
+import pathlib
+import struct
+
+import numpy as np
+import pytest
+
+import j2k_tools as jt
+import t2_oracle as t2
+
+J2K = pathlib.Path(__file__).resolve().parent / "golden" / "j2k"
+FIXTURES = sorted(p for p in J2K.iterdir() if p.suffix in (".j2k", ".jp2"))
+
+
+@pytest.fixture(scope="module")
+def images():
+    with np.load(J2K / "images.npz", allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+# ------------------------------------------------------- bit-level pins --
+
+class _BitWriter:
+    """T.800 B.10.1 packet-header bit packing: a byte after 0xFF holds 7 bits."""
+
+    def __init__(self):
+        self.out, self.cur, self.n, self.cap = bytearray(), 0, 0, 8
+
+    def bit(self, b):
+        self.cur = (self.cur << 1) | (b & 1)
+        self.n += 1
+        if self.n == self.cap:
+            self.out.append(self.cur)
+            self.cap = 7 if self.cur == 0xFF else 8
+            self.cur, self.n = 0, 0
+
+    def bits(self, v, n):
+        for i in range(n - 1, -1, -1):
+            self.bit(v >> i)
+
+    def flush(self):
+        if self.n:
+            self.out.append(self.cur << (self.cap - self.n))
+            self.cap = 7 if self.out[-1] == 0xFF else 8
+            self.cur, self.n = 0, 0
+        if self.out and self.out[-1] == 0xFF:
+            self.out.append(0)
+        return bytes(self.out)
+
+
+def _write_passes(w: _BitWriter, n: int):
+    """T.800 Table B.4 codewords."""
+    if n == 1:
+        w.bit(0)
+    elif n == 2:
+        w.bits(0b10, 2)
+    elif n <= 5:
+        w.bits(0b1100 | (n - 3), 4)
+    elif n <= 36:
+        w.bits(0b1111, 4)
+        w.bits(n - 6, 5)
+    else:
+        w.bits(0b111111111, 9)
+        w.bits(n - 37, 7)
+
+
+def test_num_passes_codeword_all_values():
+    w = _BitWriter()
+    for n in range(1, 165):
+        _write_passes(w, n)
+    data = w.flush()
+    r = t2._Bits(data + b"\0", 0)
+    assert [t2._num_passes(r) for _ in range(164)] == list(range(1, 165))
+
+
+def test_bit_reader_stuffing():
+    rng = np.random.default_rng(1)
+    bits = rng.integers(0, 2, 4000)
+    bits[:40] = 1                       # runs of ones make 0xFF bytes
+    w = _BitWriter()
+    for b in bits:
+        w.bit(int(b))
+    data = w.flush()
+    assert b"\xff" in data
+    r = t2._Bits(data + b"\0", 0)
+    assert [r.bit() for _ in range(len(bits))] == [int(b) for b in bits]
+    # after a header ending in 0xFF the stuffed byte is skipped by align
+    w2 = _BitWriter()
+    for _ in range(8):
+        w2.bit(1)
+    hdr = w2.flush()
+    assert hdr == b"\xff\x00"
+    r2 = t2._Bits(hdr + b"\xab", 0)
+    assert [r2.bit() for _ in range(8)] == [1] * 8
+    r2.align()
+    assert r2.cur == 2
+
+
+def _tag_encode(values: np.ndarray, w: _BitWriter, leaf: int, threshold: int, state):
+    """T.800 B.10.2 tag-tree encoder (independent of the decoders): node
+    values are minima of their children; `state` keeps each node's current
+    lower bound and whether its value was signalled."""
+    h, wd = values.shape
+    levels = [values.astype(np.int64)]
+    while levels[-1].shape != (1, 1):
+        v = levels[-1]
+        ph, pw = (v.shape[0] + 1) // 2, (v.shape[1] + 1) // 2
+        p = np.full((ph, pw), 1 << 30, np.int64)
+        for i in range(v.shape[0]):
+            for j in range(v.shape[1]):
+                p[i // 2, j // 2] = min(p[i // 2, j // 2], v[i, j])
+        levels.append(p)
+    i, j = divmod(leaf, wd)
+    low = 0
+    for lv in range(len(levels) - 1, -1, -1):
+        key = (lv, i >> lv, j >> lv)
+        nlow, known = state.get(key, (0, False))
+        val = int(levels[lv][i >> lv, j >> lv])
+        if low > nlow:
+            nlow = low
+        else:
+            low = nlow
+        while low < threshold:
+            if low >= val:
+                if not known:
+                    w.bit(1)
+                    known = True
+                break
+            w.bit(0)
+            low += 1
+        state[key] = (low, known)
+
+
+@pytest.mark.parametrize("wd,h", [(1, 1), (1, 5), (3, 1), (2, 2), (3, 3), (5, 4), (4, 7)])
+def test_tag_tree_brute_force(wd, h):
+    """Leaf values decoded by the oracle's tag tree equal the encoded ones,
+    for the inclusion usage (threshold raised layer by layer, leaves in
+    raster order per layer) and the zero-bit-plane usage (full decode)."""
+    rng = np.random.default_rng(wd * 100 + h)
+    for trial in range(30):
+        vals = rng.integers(0, 5, (h, wd))
+        # inclusion usage: per layer l, every leaf not yet known is asked with threshold l+1
+        w, st = _BitWriter(), {}
+        L = int(vals.max()) + 2
+        asked = []
+        for l in range(L):
+            for k in range(vals.size):
+                if vals.flat[k] >= l:       # not yet included before this layer
+                    _tag_encode(vals, w, k, l + 1, st)
+                    asked.append((l, k))
+        data = w.flush() + b"\0" * 8
+        tree, r = t2._TagTree(wd, h), t2._Bits(data, 0)
+        got = {}
+        for l, k in asked:
+            if tree.decode(r, k, l + 1) and k not in got:
+                got[k] = l
+        assert [got[k] for k in range(vals.size)] == list(vals.flat), vals
+        # zero-bit-plane usage: each leaf decoded to its value in raster order
+        w, st = _BitWriter(), {}
+        for k in range(vals.size):
+            _tag_encode(vals, w, k, 1 << 20, st)
+        data = w.flush() + b"\0" * 8
+        tree, r = t2._TagTree(wd, h), t2._Bits(data, 0)
+        out = []
+        for k in range(vals.size):
+            tree.decode(r, k, 1 << 62)
+            out.append(tree.value[k])
+        assert out == list(vals.flat)
+
+
+# ------------------------------------------------ oracle vs OpenJPEG data --
+
+def _rct_dc(planes, mct, depth):
+    """T.800 G.2 inverse RCT on components 0..2 (when mct) and DC shift."""
+    p = [np.asarray(x).astype(np.int64) for x in planes]
+    if mct:
+        y0, y1, y2 = p[:3]
+        g = y0 - ((y1 + y2) >> 2)
+        p[:3] = [y2 + g, g, y1 + g]
+    return [x + (1 << (depth - 1)) for x in p]
+
+
+def _ict_dc(planes, mct, depth):
+    """T.800 G.3 inverse ICT (float64) and DC shift, rounded to nearest."""
+    p = [np.asarray(x).view(np.float32).astype(np.float64) for x in planes]
+    if mct:
+        y, cb, cr = p[:3]
+        p[:3] = [y + 1.402 * cr, y - 0.344136 * cb - 0.714136 * cr, y + 1.772 * cb]
+    return [np.clip(np.floor(x + (1 << (depth - 1)) + 0.5), 0, (1 << depth) - 1) for x in p]
+
+
+LOSSLESS = [p for p in FIXTURES if "97" not in p.name]
+LOSSY = [p for p in FIXTURES if "97" in p.name]
+
+
+@pytest.mark.parametrize("path", LOSSLESS, ids=lambda p: p.stem)
+def test_oracle_decodes_lossless_fixture_exactly(orc, images, path):
+    data = path.read_bytes()
+    P = t2.parse(data)
+    planes = t2.decode_image(data, orc, out_format=7, r53_mode=orc.R53_TEXTBOOK)
+    depth = P["siz"]["comps"][0]["depth"]
+    got = np.stack(_rct_dc(planes, P["tiles"][0]["mct"], depth), axis=-1)
+    src = images[path.stem]
+    assert np.array_equal(got, src.reshape(got.shape)), path.stem
+
+
+def _single_sample_tiles(P):
+    """Tiles in which some synthesised resolution (r >= 1) is one sample
+    wide or high.  The reference's 9/7 row pass outputs such a line as
+    low * lowGain (HDWT_R97, bmi_cpu_dwt_decoder.cpp:2300-2303), keeping the
+    K normalisation that T.800 F.3.7 drops for a single sample, so these
+    tiles differ from a T.800 decoder by up to a factor K."""
+    out = []
+    for T in P["tiles"]:
+        for r in range(1, T["levels"] + 1):
+            s = T["levels"] - r
+            w = -(-T["tx1"] // (1 << s)) + (T["tx0"] // -(1 << s))
+            h = -(-T["ty1"] // (1 << s)) + (T["ty0"] // -(1 << s))
+            if w == 1 or h == 1:
+                out.append(T)
+                break
+    return out
+
+
+@pytest.mark.parametrize("path", LOSSY, ids=lambda p: p.stem)
+def test_oracle_97_t800_placement_matches_openjpeg(orc, images, path):
+    """With the T.800 bit-plane placement the oracle's 9/7 decode is within
+    2 levels of OpenJPEG's (different float arithmetic and rounding) outside
+    tiles with single-sample resolutions; with the reference's numbps
+    (guard bits 2 here) it is not."""
+    data = path.read_bytes()
+    P = t2.parse(data)
+    depth, mct = P["siz"]["comps"][0]["depth"], P["tiles"][0]["mct"]
+    opj = images[path.stem + "_opj"].astype(np.int64)
+    got = np.stack(_ict_dc(t2.decode_image(data, orc, out_format=7, guard_t800=True), mct, depth),
+                   axis=-1).reshape(opj.shape)
+    mask = np.ones(opj.shape[:2], bool)
+    for T in _single_sample_tiles(P):
+        oy, ox = T["ty0"] - P["siz"]["YOsiz"], T["tx0"] - P["siz"]["XOsiz"]
+        mask[oy:oy + T["ty1"] - T["ty0"], ox:ox + T["tx1"] - T["tx0"]] = False
+        # the quirk is really there: the tile is off by far more than rounding
+        sub = np.abs(got - opj)[oy:oy + T["ty1"] - T["ty0"], ox:ox + T["tx1"] - T["tx0"]]
+        assert sub.max() > 8
+    assert mask.mean() > 0.9
+    d = np.abs(got - opj)[mask]
+    assert d.max() <= 2
+    assert d.mean() < 0.6
+    ref = np.stack(_ict_dc(t2.decode_image(data, orc, out_format=7), mct, depth),
+                   axis=-1).reshape(opj.shape)
+    assert np.abs(ref - opj)[mask].max() > 8
+
+
+# --------------------------------------------------- product parse parity --
+
+def _compare(pkg, data: bytes, flags: int = 0) -> int:
+    """Product parse (C++ in libbmi_j2k.so) == oracle parse; returns blocks."""
+    P = t2.parse(data, guard_t800=bool(flags))
+    cs = pkg.Codestream(data, flags)
+    assert cs.ntiles == len(P["tiles"])
+    total = 0
+    for t, T in enumerate(P["tiles"]):
+        x0, y0, w, h, nb, sb = cs.tile_info(t)
+        assert (x0, y0, w, h) == (T["tx0"], T["ty0"], T["tx1"] - T["tx0"], T["ty1"] - T["ty0"])
+        assert nb == len(T["blocks"])
+        base = 1 << 40
+        ptrs = [base * (c + 1) for c in range(T["ncomp"])]
+        pitch = w + 3
+        blocks, segs, desc = cs.tile_blocks(t, ptrs, pitch, seg_base=0)
+        assert sb == sum(len(b["data"]) for b in T["blocks"])
+        assert segs[sb:sb + 16].tobytes() == b"\xff" * 16
+        prod = {}
+        for i in range(nb):
+            b = blocks[i]
+            c = (b.out // base) - 1
+            y, x = divmod((b.out - ptrs[c]) // 4, pitch)
+            assert b.out_pitch == pitch
+            prod[(c, x, y)] = (b.width, b.height, b.orient, b.numbps, b.missing, b.num_pass,
+                               b.kind, segs[b.data_offset:b.data_offset + b.len].tobytes())
+        kind = 1 if T["filt"] == "53" else 0
+        orc_tab = {(b["comp"], b["x"], b["y"]): (b["w"], b["h"], b["orient"], b["numbps"],
+                                                  b["missing"], b["num_pass"], kind, b["data"])
+                   for b in T["blocks"]}
+        assert prod == orc_tab
+        L = T["levels"]
+        steps = np.array([[desc.steps[c][k] for k in range(3 * L + 1)] for c in range(T["ncomp"])],
+                         np.float32)
+        assert np.array_equal(steps, np.asarray(T["steps"], np.float32))
+        assert (desc.width, desc.height, desc.x0, desc.y0) == (w, h, x0, y0)
+        assert (desc.levels, desc.ncomp, desc.method) == (L, T["ncomp"], 0 if kind else 1)
+        assert desc.mct == int(bool(T["mct"]) and T["ncomp"] >= 3)
+        assert [desc.coef[c] for c in range(T["ncomp"])] == ptrs
+        total += nb
+    cs.close()
+    return total
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=lambda p: p.stem)
+def test_product_parse_matches_oracle(pkg, path):
+    data = path.read_bytes()
+    assert _compare(pkg, data, pkg.CS_REFERENCE) > 0
+    _compare(pkg, data, pkg.CS_GUARD_T800)
+
+
+REWRITES = [dict(sop=True), dict(eph=True), dict(sop=True, eph=True, parts=3),
+            dict(parts=2, comment_in_tile=True), dict(keep_packets=5), dict(keep_packets=1)]
+
+
+@pytest.mark.parametrize("path", [J2K / "rgb_tiles_offsets.j2k", J2K / "grey_layers_lrcp.j2k",
+                                  J2K / "rgb97_tiles_pcrl.j2k", J2K / "rgb_cprl_4lev.j2k"],
+                         ids=lambda p: p.stem)
+def test_product_parse_rewritten_streams(pkg, orc, path):
+    """SOP / EPH markers, tile-parts and truncation: product == oracle, and
+    marker insertion / tile-part splits decode to the same planes."""
+    data = path.read_bytes()
+    base = t2.decode_image(data, orc, out_format=7, r53_mode=orc.R53_TEXTBOOK)
+    for kw in REWRITES:
+        d2 = jt.rewrite(data, **kw)
+        _compare(pkg, d2)
+        if "keep_packets" not in kw:
+            got = t2.decode_image(d2, orc, out_format=7, r53_mode=orc.R53_TEXTBOOK)
+            assert all(np.array_equal(a, b) for a, b in zip(base, got)), kw
+
+
+def test_codestream_info(pkg):
+    cs = pkg.Codestream((J2K / "rgb_tiles_offsets.j2k").read_bytes())
+    i = cs.info
+    assert (i.width, i.height, i.x0, i.y0) == (90, 100, 9, 13)
+    assert (i.tile_w, i.tile_h, i.tile_x0, i.tile_y0) == (32, 32, 7, 11)
+    assert (i.ntiles_x, i.ntiles_y, i.ncomp, i.bit_depth) == (3, 4, 3, 8)
+    assert (i.levels, i.reversible, i.mct, i.layers, i.cb_w, i.cb_h) == (5, 1, 1, 1, 64, 64)
+    assert i.colour_space == 0
+    cs2 = pkg.Codestream((J2K / "rgb_nomct_jp2.jp2").read_bytes())
+    assert cs2.info.colour_space == 16 and cs2.info.mct == 0
+    cs3 = pkg.Codestream((J2K / "grey_pcrl_cb16x32.j2k").read_bytes())
+    assert (cs3.info.progression, cs3.info.cb_w, cs3.info.cb_h) == (3, 16, 32)
+
+
+def _expect_error(pkg, data: bytes, code: int):
+    with pytest.raises(pkg.BmiError) as e:
+        pkg.Codestream(data)
+    assert f"({code})" in str(e.value), str(e.value)
+
+
+def _set_cod_byte(data: bytes, offset: int, value: int) -> bytes:
+    hdr = jt.main_header(data)
+    pos = 2
+    while t2._u(hdr, pos, 2) != t2.MK_COD:
+        pos += 2 + t2._u(hdr, pos + 2, 2)
+    b = bytearray(data)
+    b[pos + 4 + offset] = value
+    return bytes(b)
+
+
+def test_product_rejects_malformed_and_unsupported(pkg):
+    data = (J2K / "grey_layers_lrcp.j2k").read_bytes()
+    _expect_error(pkg, b"", pkg.BMI_E_INVALID)
+    _expect_error(pkg, b"\x00\x01\x02\x03garbage", pkg.BMI_E_INVALID)
+    _expect_error(pkg, data[:30], pkg.BMI_E_INVALID)                  # inside the main header
+    hdr = jt.main_header(data)
+    _expect_error(pkg, data[:len(hdr) + 20], pkg.BMI_E_INVALID)      # inside a tile-part header
+    # a packet body cut short (tile-part length kept)
+    body_cut = jt.rewrite(data)
+    _expect_error(pkg, body_cut[:-40] + b"\xff\xd9", pkg.BMI_E_INVALID)
+    for marker in (t2.MK_POC, t2.MK_RGN, t2.MK_PPM):
+        payload = b"\x00" * (7 if marker == t2.MK_POC else 3)
+        bad = jt.rewrite(data, extra_main=struct.pack(">HH", marker, len(payload) + 2) + payload)
+        _expect_error(pkg, bad, pkg.BMI_E_UNSUPPORTED)
+    # code-block styles: bypass (1), reset (2), termall (4), causal (8), segsym (32)
+    for style in (1, 2, 4, 8, 32):
+        _expect_error(pkg, _set_cod_byte(data, 8, style), pkg.BMI_E_UNSUPPORTED)
+    # predictable termination changes nothing for the decoder
+    assert pkg.Codestream(_set_cod_byte(data, 8, 16)).info.total_blocks > 0
+    # 128x32 code-blocks: legal (area 4096), wider than the GPU decoder's 64
+    _expect_error(pkg, _set_cod_byte(_set_cod_byte(data, 7, 3), 6, 5), pkg.BMI_E_UNSUPPORTED)
+    # 128x64: area over 4096, malformed
+    _expect_error(pkg, _set_cod_byte(data, 6, 5), pkg.BMI_E_INVALID)
+    # COM in the main header is skipped
+    com = struct.pack(">HH", t2.MK_COM, 8) + b"\x00\x01test"
+    assert pkg.Codestream(jt.rewrite(data, extra_main=com)).info.total_blocks > 0
+
+
+def test_tile_blocks_argument_checks(pkg):
+    cs = pkg.Codestream((J2K / "grey_37x53.j2k").read_bytes())
+    _, _, w, h, nb, sb = cs.tile_info(0)
+    with pytest.raises(pkg.BmiError):
+        cs.tile_blocks(0, [1 << 40], w - 1)                 # pitch below the width
+    with pytest.raises(pkg.BmiError):
+        cs.tile_blocks(0, [1 << 40], w, segs=np.zeros(sb + 15, np.uint8))   # no 16-byte slack
+    with pytest.raises(pkg.BmiError):
+        cs.tile_info(1)
+    blocks, segs, desc = cs.tile_blocks(0, [1 << 40], w)
+    pkg.t1_check_blocks([blocks[i] for i in range(nb)])
+
+
+def test_product_parse_random_openjpeg_streams(pkg, orc):
+    """Seeded sweep of encoder settings (needs Pillow with OpenJPEG, which
+    this image has): product == oracle, lossless streams exact."""
+    Image = pytest.importorskip("PIL.Image")
+    rng = np.random.default_rng(77)
+    progs = ["LRCP", "RLCP", "RPCL", "PCRL", "CPRL"]
+    for trial in range(8):
+        h, w = int(rng.integers(1, 90)), int(rng.integers(1, 90))
+        nc = int(rng.choice([1, 3]))
+        nres = int(rng.integers(1, 6))
+        opts = dict(progression=progs[trial % 5], num_resolutions=nres)
+        if rng.random() < 0.5:
+            opts["tile_size"] = (int(rng.integers(16, 64)), int(rng.integers(16, 64)))
+        if rng.random() < 0.5:
+            opts["codeblock_size"] = (int(2 ** rng.integers(2, 7)), int(2 ** rng.integers(2, 7)))
+            if opts["codeblock_size"][0] * opts["codeblock_size"][1] > 4096:
+                opts["codeblock_size"] = (64, 64)
+        if rng.random() < 0.5:
+            opts["quality_layers"] = [40, 10, 0]
+        if nc == 3:
+            opts["mct"] = int(rng.integers(0, 2))
+        img = rng.integers(0, 256, (h, w, nc) if nc == 3 else (h, w), dtype=np.uint8)
+        b = io.BytesIO()
+        try:
+            Image.fromarray(img).save(b, format="JPEG2000", no_jp2=True, **opts)
+        except OSError:
+            continue           # settings the encoder refuses for this size
+        data = b.getvalue()
+        _compare(pkg, data)
+        P = t2.parse(data)
+        planes = t2.decode_image(data, orc, out_format=7, r53_mode=orc.R53_TEXTBOOK)
+        got = np.stack(_rct_dc(planes, P["tiles"][0]["mct"], 8), axis=-1)
+        assert np.array_equal(got, img.reshape(got.shape)), opts

@@ -1,0 +1,117 @@
+"""GPU decode of JPEG 2000 codestreams end to end: the product's host tier-2
+parser (C++), one GPU tier-1 launch over every code-block of every tile and
+one batched IDWT / colour plan over all tiles.
+
+  - every fixture (streams written by OpenJPEG, tests/golden/j2k) decodes
+    bit-exactly to the oracle's whole-image decode in the packed output
+    format (reference 5/3 mode, reference numbps);
+  - lossless streams decode back to the source image exactly with the
+    T.800 5/3 synthesis (component planes, inverse RCT and DC shift applied
+    here);
+  - lossy streams with BMI_CS_GUARD_T800 equal the oracle's T.800-placement
+    planes bit for bit;
+  - streams rewritten with SOP / EPH markers, tile-parts and truncated
+    tiles decode as the oracle does; a decoder object re-runs to the same
+    output."""
+import pathlib
+
+import numpy as np
+import pytest
+
+import j2k_tools as jt
+import t2_oracle as t2
+
+J2K = pathlib.Path(__file__).resolve().parent / "golden" / "j2k"
+FIXTURES = sorted(p for p in J2K.iterdir() if p.suffix in (".j2k", ".jp2"))
+LOSSLESS = [p for p in FIXTURES if "97" not in p.name]
+LOSSY = [p for p in FIXTURES if "97" in p.name]
+
+
+@pytest.fixture(scope="module")
+def images():
+    with np.load(J2K / "images.npz", allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def _gpu_packed(pkg, data, **kw):
+    out = pkg.decode_codestream(data, **kw)
+    return out.cpu().numpy()
+
+
+def _gpu_planes(pkg, data, **kw):
+    planes = pkg.decode_codestream(data, out_format=pkg.FMT_NONE, **kw)
+    return [p.cpu().numpy() for p in planes]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", FIXTURES, ids=lambda p: p.stem)
+def test_gpu_codestream_matches_oracle(pkg, orc, path):
+    import torch
+    assert torch.cuda.is_available()
+    data = path.read_bytes()
+    print(f"case {path.stem}", flush=True)
+    cs = pkg.Codestream(data)
+    fmt, co = pkg.default_format(cs.info)
+    got = _gpu_packed(pkg, data)
+    exp = t2.decode_image(data, orc, out_format=fmt, check_out=co)
+    assert got.shape == exp.shape
+    assert np.array_equal(got, exp), (path.stem, np.count_nonzero(got != exp))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", LOSSLESS, ids=lambda p: p.stem)
+def test_gpu_lossless_codestream_is_exact(pkg, images, path):
+    data = path.read_bytes()
+    print(f"case {path.stem}", flush=True)
+    cs = pkg.Codestream(data)
+    planes = _gpu_planes(pkg, data, r53_mode=pkg.R53_TEXTBOOK)
+    p = [x.astype(np.int64) for x in planes]
+    if cs.info.mct and cs.info.ncomp >= 3:
+        y0, y1, y2 = p[:3]
+        g = y0 - ((y1 + y2) >> 2)
+        p[:3] = [y2 + g, g, y1 + g]
+    got = np.stack([x + (1 << (cs.info.bit_depth - 1)) for x in p], axis=-1)
+    src = images[path.stem]
+    assert np.array_equal(got, src.reshape(got.shape)), path.stem
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", LOSSY, ids=lambda p: p.stem)
+def test_gpu_codestream_guard_t800(pkg, orc, path):
+    data = path.read_bytes()
+    print(f"case {path.stem}", flush=True)
+    got = _gpu_planes(pkg, data, flags=pkg.CS_GUARD_T800)
+    exp = t2.decode_image(data, orc, out_format=7, guard_t800=True)
+    for c, (g, e) in enumerate(zip(got, exp)):
+        assert np.array_equal(g.view(np.uint32), e.view(np.uint32)), (path.stem, c)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kw", [dict(sop=True, eph=True, parts=3), dict(parts=2, comment_in_tile=True),
+                                dict(keep_packets=5)], ids=["sop_eph_parts", "parts_com", "truncated"])
+def test_gpu_codestream_rewritten(pkg, orc, kw):
+    for name in ("rgb97_tiles_pcrl.j2k", "grey_layers_lrcp.j2k"):
+        data = jt.rewrite((J2K / name).read_bytes(), **kw)
+        print(f"case {name} {kw}", flush=True)
+        cs = pkg.Codestream(data)
+        fmt, co = pkg.default_format(cs.info)
+        got = _gpu_packed(pkg, data)
+        exp = t2.decode_image(data, orc, out_format=fmt, check_out=co)
+        assert np.array_equal(got, exp), (name, kw)
+
+
+@pytest.mark.gpu
+def test_gpu_codestream_decoder_reruns(pkg):
+    import torch
+    data = (J2K / "rgb97_256.j2k").read_bytes()
+    dec = pkg.CodestreamDecoder(data)
+    dec.run(0)
+    torch.cuda.synchronize()
+    first = dec.out.cpu().numpy().copy()
+    dec.out.zero_()
+    for _ in range(3):
+        dec.run(0)
+    torch.cuda.synchronize()
+    assert np.array_equal(dec.out.cpu().numpy(), first)
+    assert dec.plan is not None and dec.plan.error_flag() == 0
+    dec.close()
