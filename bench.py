@@ -290,7 +290,66 @@ def t1_leg(pkg, wl, torch, dev, stream, steps, warmup):
             "ms_per_image": round(ms, 3), "Mpixel_s": round(W * H / (ms * 1e-3) / 1e6, 1),
             "t1_ms": round(t1_ms, 3), "t1_Mcoef_s": round(ncoef / (t1_ms * 1e-3) / 1e6, 1),
             "t1_blocks_s": round(nb / (t1_ms * 1e-3), 0),
-            "encode_s_host": round(enc_s, 2), "_img": img}
+            "encode_s_host": round(enc_s, 2), "_img": img, "_out": out}
+
+
+def codestream_leg(pkg, wl, torch, dev, stream, steps, warmup, img, ref_out):
+    """From a JPEG 2000 file in host memory to pixels: the tier-1 leg's
+    code-blocks wrapped in a codestream (one tile, one layer, LRCP, default
+    precincts; workloads.t1_codestream).  Host tier 2 (BMI_codestream_open +
+    BMI_codestream_tile_blocks into pinned memory), one H2D copy of the
+    code-block segments, then GPU tier 1 and the fused IDWT + ICT; each part
+    timed on its own and the output checked against the tier-1 leg's."""
+    W, H = img["w"], img["h"]
+    t0 = time.perf_counter()
+    data = wl.t1_codestream(img)
+    write_s = time.perf_counter() - t0
+    dec = pkg.CodestreamDecoder(data, out_format=pkg.FMT_24BITS_RGB)
+    sh = stream.cuda_stream
+    _, _, w, h, nb, sb = dec.cs.tile_info(0)
+    pinned = torch.empty(sb + 16, dtype=torch.uint8, pin_memory=True)
+    pn = pinned.numpy()
+    ptrs = [dec.planes[0][c].data_ptr() for c in range(3)]
+    pitch = dec.planes[0].shape[2]
+    host = []
+    for _ in range(7):
+        a = time.perf_counter()
+        cs = pkg.Codestream(data)
+        b = time.perf_counter()
+        cs.tile_blocks(0, ptrs, pitch, 0, segs=pn)
+        c = time.perf_counter()
+        cs.close()
+        host.append((b - a, c - b))
+    host.sort(key=lambda x: x[0] + x[1])
+    parse_s, blocks_s = host[len(host) // 2]
+    assert pn[:sb].tobytes() == dec.d_segs[:sb].cpu().numpy().tobytes()
+    for _ in range(warmup):
+        dec.run(sh)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(stream)
+    for _ in range(steps):
+        dec.d_segs.copy_(pinned, non_blocking=True)
+    ev[1].record(stream)
+    ev[2].record(stream)
+    for _ in range(steps):
+        dec.run(sh)
+    ev[3].record(stream)
+    torch.cuda.synchronize()
+    h2d_ms = ev[0].elapsed_time(ev[1]) / steps
+    gpu_ms = ev[2].elapsed_time(ev[3]) / steps
+    same = bool(torch.equal(dec.out, ref_out))
+    dec.close()
+    host_ms = (parse_s + blocks_s) * 1e3
+    e2e = host_ms + h2d_ms + gpu_ms
+    return {"workload": f"{W}x{H}x3 9/7 L=5 .j2k, {nb} code-blocks, {len(data)} bytes "
+                        "(one tile, one layer, LRCP)",
+            "host_parse_ms": round(parse_s * 1e3, 3), "host_tile_blocks_ms": round(blocks_s * 1e3, 3),
+            "h2d_ms": round(h2d_ms, 3), "h2d_GBs": round(sb / (h2d_ms * 1e-3) / 1e9, 1),
+            "gpu_ms": round(gpu_ms, 3), "gpu_Mpixel_s": round(W * H / (gpu_ms * 1e-3) / 1e6, 1),
+            "serial_file_to_pixels_ms": round(e2e, 3),
+            "serial_file_to_pixels_Mpixel_s": round(W * H / (e2e * 1e-3) / 1e6, 1),
+            "output_equals_t1_leg": same, "write_s_host": round(write_s, 2)}
 
 
 def t1_cpu_baseline(img, budget_s: float = 8.0):
@@ -493,10 +552,17 @@ def main() -> int:
         serving = serving_leg(pkg, wl, torch, dev, rates, args.serve_requests)
 
     t1 = None
+    codestream = None
     if not args.no_t1:
         log("tier 1: GPU code-block decode + fused IDWT/ICT")
         t1 = t1_leg(pkg, wl, torch, dev, stream, max(5, args.steps // 5), 2)
         log(f"t1: {t1['t1_ms']} ms ({t1['t1_Mcoef_s']} Mcoef/s), image {t1['ms_per_image']} ms")
+        log("codestream: host tier 2 + H2D + GPU tier 1 + IDWT/ICT")
+        codestream = codestream_leg(pkg, wl, torch, dev, stream, max(5, args.steps // 5), 2,
+                                    t1["_img"], t1["_out"])
+        log(f"codestream: host {codestream['host_parse_ms']}+{codestream['host_tile_blocks_ms']} ms, "
+            f"h2d {codestream['h2d_ms']} ms, gpu {codestream['gpu_ms']} ms, "
+            f"equal={codestream['output_equals_t1_leg']}")
 
     cpu = None
     if not args.no_cpu:
@@ -508,6 +574,7 @@ def main() -> int:
             log(f"cpu t1: {cpu['t1']['value']} Mcoef/s")
     if t1 is not None:
         t1.pop("_img", None)
+        t1.pop("_out", None)
 
     rec = {
         "metric": METRIC,
@@ -536,6 +603,7 @@ def main() -> int:
         "batched": batched,
         "serving": serving,
         "t1": t1,
+        "codestream": codestream,
     }
     plan.close()
     print(json.dumps(rec), flush=True)

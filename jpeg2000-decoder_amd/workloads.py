@@ -292,3 +292,153 @@ def t1_image(w: int, h: int, levels: int, ncomp: int, seed: int = 0, cb: int = 6
         off += len(data)
     return dict(q=q, blocks=blocks, data=b"".join(chunks) + b"\xff" * 16, steps=steps,
                 filt=filt, levels=levels, w=w, h=h, ncomp=ncomp)
+
+
+# A JPEG 2000 codestream around the code-blocks of t1_image: one tile, one
+# layer, LRCP, default precincts, guard bits 1 (ITU-T T.800 Annex A markers,
+# B.10 packet headers), so the codestream front end decodes exactly the
+# blocks the tier-1 workload holds.
+
+class _HeaderBits:
+    """Packet-header bit packing: a byte after 0xFF carries 7 bits."""
+
+    def __init__(self):
+        self.buf, self.acc, self.n, self.cap = bytearray(), 0, 0, 8
+
+    def put(self, bit: int):
+        self.acc = (self.acc << 1) | (bit & 1)
+        self.n += 1
+        if self.n == self.cap:
+            self.buf.append(self.acc)
+            self.cap = 7 if self.acc == 0xFF else 8
+            self.acc, self.n = 0, 0
+
+    def put_bits(self, v: int, n: int):
+        for i in range(n - 1, -1, -1):
+            self.put(v >> i)
+
+    def finish(self) -> bytes:
+        if self.n:
+            self.buf.append(self.acc << (self.cap - self.n))
+        if self.buf and self.buf[-1] == 0xFF:
+            self.buf.append(0)
+        return bytes(self.buf)
+
+
+class _TagTreeWriter:
+    """Tag-tree encoder over a leaf grid (node value = minimum of its
+    children; per node the current lower bound and whether it is known)."""
+
+    def __init__(self, vals: np.ndarray):
+        self.lv = [np.asarray(vals, dtype=np.int64)]
+        while self.lv[-1].shape != (1, 1):
+            v = self.lv[-1]
+            h, w = v.shape
+            p = np.full(((h + 1) // 2, (w + 1) // 2), 1 << 40, dtype=np.int64)
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    s = v[dy::2, dx::2]
+                    p[:s.shape[0], :s.shape[1]] = np.minimum(p[:s.shape[0], :s.shape[1]], s)
+            self.lv.append(p)
+        self.low = [np.zeros(v.shape, np.int64) for v in self.lv]
+        self.known = [np.zeros(v.shape, bool) for v in self.lv]
+
+    def encode(self, bits: _HeaderBits, i: int, j: int, threshold: int):
+        low = 0
+        for k in range(len(self.lv) - 1, -1, -1):
+            a, b = i >> k, j >> k
+            if low > self.low[k][a, b]:
+                self.low[k][a, b] = low
+            else:
+                low = int(self.low[k][a, b])
+            val = int(self.lv[k][a, b])
+            while low < threshold:
+                if low >= val:
+                    if not self.known[k][a, b]:
+                        bits.put(1)
+                        self.known[k][a, b] = True
+                    break
+                bits.put(0)
+                low += 1
+            self.low[k][a, b] = low
+
+
+def _pass_codeword(bits: _HeaderBits, n: int):
+    if n == 1:
+        bits.put(0)
+    elif n == 2:
+        bits.put_bits(0b10, 2)
+    elif n <= 5:
+        bits.put_bits(0b1100 | (n - 3), 4)
+    elif n <= 36:
+        bits.put_bits(0b1111, 4)
+        bits.put_bits(n - 6, 5)
+    else:
+        bits.put_bits(0b111111111, 9)
+        bits.put_bits(n - 37, 7)
+
+
+def t1_codestream(img: dict, cb: int = 64) -> bytes:
+    """Raw codestream (.j2k) holding the code-blocks of t1_image(img): one
+    tile, one layer, LRCP, default precincts, code-block style 0."""
+    import struct
+    w, h, L, nc, filt = img["w"], img["h"], img["levels"], img["ncomp"], img["filt"]
+    data = img["data"]
+    by_band = {}
+    for b in img["blocks"]:
+        by_band.setdefault((b["comp"], b["band"]), []).append(b)
+    numbps = {b["band"]: b["numbps"] for b in img["blocks"]}
+    out = bytearray(struct.pack(">H", 0xFF4F))
+    siz = struct.pack(">HIIIIIIIIH", 0, w, h, 0, 0, w, h, 0, 0, nc) + bytes([7, 1, 1]) * nc
+    out += struct.pack(">HH", 0xFF51, 2 + len(siz)) + siz
+    lg = cb.bit_length() - 1
+    cod = struct.pack(">BBHBBBBBB", 0, 0, 1, 1 if nc == 3 else 0, L, lg - 2, lg - 2, 0,
+                      0 if filt == "97" else 1)
+    out += struct.pack(">HH", 0xFF52, 2 + len(cod)) + cod
+    if filt == "97":        # guard bits 1: QCD value = numbps
+        qcd = bytes([(1 << 5) | 2]) + b"".join(struct.pack(">H", numbps[k]) for k in range(3 * L + 1))
+    else:                   # exponent = numbps with guard bits 1
+        qcd = bytes([(1 << 5) | 0]) + bytes(numbps[k] << 3 for k in range(3 * L + 1))
+    out += struct.pack(">HH", 0xFF5C, 2 + len(qcd)) + qcd
+    packets = bytearray()
+    for r in range(L + 1):
+        for c in range(nc):
+            bands = [0] if r == 0 else [3 * (r - 1) + o for o in (1, 2, 3)]
+            hb, body, any_incl = _HeaderBits(), bytearray(), False
+            plan = []
+            for bid in bands:
+                blks = sorted(by_band.get((c, bid), []), key=lambda b: (b["y"], b["x"]))
+                if not blks:
+                    plan.append(None)
+                    continue
+                xs = sorted({b["x"] for b in blks})
+                ys = sorted({b["y"] for b in blks})
+                grid = np.array([[1 if b["num_pass"] == 0 else 0 for b in blks]]).reshape(len(ys), len(xs))
+                zbp = np.array([b["missing"] if b["num_pass"] else 1 << 20 for b in blks]).reshape(grid.shape)
+                plan.append((blks, len(xs), _TagTreeWriter(grid), _TagTreeWriter(zbp)))
+                any_incl |= bool((grid == 0).any())
+            if not any_incl:
+                hb.put(0)
+            else:
+                hb.put(1)
+                for p in plan:
+                    if p is None:
+                        continue
+                    blks, gw, incl, zbp = p
+                    for k, b in enumerate(blks):
+                        i, j = divmod(k, gw)
+                        incl.encode(hb, i, j, 1)
+                        if b["num_pass"] == 0:
+                            continue
+                        zbp.encode(hb, i, j, 1 << 30)
+                        _pass_codeword(hb, b["num_pass"])
+                        base = 3 + b["num_pass"].bit_length() - 1
+                        extra = max(0, max(1, b["len"].bit_length()) - base)
+                        hb.put_bits((1 << (extra + 1)) - 2, extra + 1)   # extra ones, then 0
+                        hb.put_bits(b["len"], base + extra)
+                        body += data[b["offset"]:b["offset"] + b["len"]]
+            packets += hb.finish() + body
+    sot_len = 12 + 2 + len(packets)
+    out += struct.pack(">HHHIBB", 0xFF90, 10, 0, sot_len, 0, 1)
+    out += struct.pack(">H", 0xFF93) + packets + struct.pack(">H", 0xFFD9)
+    return bytes(out)

@@ -442,3 +442,21 @@ def test_product_parse_random_openjpeg_streams(pkg, orc):
         planes = t2.decode_image(data, orc, out_format=7, r53_mode=orc.R53_TEXTBOOK)
         got = np.stack(_rct_dc(planes, P["tiles"][0]["mct"], 8), axis=-1)
         assert np.array_equal(got, img.reshape(got.shape)), opts
+
+
+@pytest.mark.parametrize("w,h,levels,nc,filt,cb", [(256, 192, 5, 3, "97", 64), (200, 130, 4, 1, "53", 32),
+                                                   (97, 61, 3, 3, "53", 16)])
+def test_own_writer_round_trip(pkg, workloads, w, h, levels, nc, filt, cb):
+    """workloads.t1_codestream (the benchmark's .j2k writer) wraps the tier-1
+    workload's code-blocks; both parsers give those blocks back unchanged."""
+    img = workloads.t1_image(w, h, levels, nc, seed=3, filt=filt, cb=cb)
+    data = workloads.t1_codestream(img, cb)
+    exp = {(b["comp"], b["x"], b["y"]): (b["w"], b["h"], b["orient"], b["numbps"], b["missing"],
+                                         b["num_pass"], img["data"][b["offset"]:b["offset"] + b["len"]])
+           for b in img["blocks"]}
+    T = t2.parse(data)["tiles"][0]
+    got = {(b["comp"], b["x"], b["y"]): (b["w"], b["h"], b["orient"], b["numbps"], b["missing"],
+                                         b["num_pass"], b["data"]) for b in T["blocks"]}
+    assert got == exp
+    assert np.array_equal(np.asarray(T["steps"], np.float32), img["steps"])
+    _compare(pkg, data)

@@ -115,3 +115,22 @@ def test_gpu_codestream_decoder_reruns(pkg):
     assert np.array_equal(dec.out.cpu().numpy(), first)
     assert dec.plan is not None and dec.plan.error_flag() == 0
     dec.close()
+
+
+@pytest.mark.gpu
+def test_gpu_own_writer_codestream_equals_t1_path(pkg, orc, workloads):
+    """The benchmark's codestream (tier-1 workload wrapped by
+    workloads.t1_codestream) decodes to the oracle's result on its blocks."""
+    img = workloads.t1_image(512, 384, 5, 3, seed=5, filt="97")
+    data = workloads.t1_codestream(img)
+    got = _gpu_packed(pkg, data, out_format=pkg.FMT_24BITS_RGB)
+    exp = t2.decode_image(data, orc, out_format=pkg.FMT_24BITS_RGB)
+    assert np.array_equal(got, exp)
+    planes = np.zeros((3, 384, 512), np.int32)
+    for b in img["blocks"]:
+        words = orc.t1_decode_block(img["data"][b["offset"]:b["offset"] + b["len"]], b["w"], b["h"],
+                                    b["orient"], b["numbps"], b["missing"], b["num_pass"])
+        planes[b["comp"], b["y"]:b["y"] + b["h"], b["x"]:b["x"] + b["w"]] = words
+    _, packed = orc.decode_tile(planes, levels=5, filt="97", out_format=pkg.FMT_24BITS_RGB,
+                                steps=img["steps"], coef_kind=orc.COEF_SIGNMAG)
+    assert np.array_equal(got, packed)
