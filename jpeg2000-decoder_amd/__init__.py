@@ -404,8 +404,16 @@ def tile_geometry(w: int, h: int, x0: int, y0: int, levels: int):
 
 
 def output_size(w: int, h: int, x0: int, y0: int, levels: int, top_level: int = -1):
+    """Output size of a tile decoded to `top_level` synthesis levels (-1:
+    all; 0: the thumbnail, i.e. the LL band)."""
     if top_level < 0 or top_level >= levels:
         return w, h
+    if top_level == 0:
+        s = 1 << levels
+
+        def cd(a):
+            return -(-a // s)
+        return cd(x0 + w) - cd(x0), cd(y0 + h) - cd(y0)
     g = tile_geometry(w, h, x0, y0, levels)[top_level - 1]
     return g["nw"], g["nh"]
 
@@ -570,7 +578,11 @@ class CodestreamDecoder:
     (e.g. timed) on the resident inputs."""
 
     def __init__(self, data: bytes, *, flags: int = CS_REFERENCE, out_format=None, check_out=None,
-                 r53_mode: int = R53_REFERENCE, device: str = "cuda"):
+                 r53_mode: int = R53_REFERENCE, top_level: int = -1, device: str = "cuda"):
+        """top_level: synthesis levels to run (-1 all; k < levels gives the
+        image at 2^-(levels-k) of full size, 0 the LL thumbnail, the
+        reference's mDwtDecodeTopLevel / GetThumbnail).  Tier 1 then decodes
+        only the code-blocks of the bands that resolution needs."""
         import numpy as np
         import torch
         self.cs = cs = Codestream(data, flags)
@@ -578,7 +590,13 @@ class CodestreamDecoder:
         fmt, co = default_format(info)
         self.out_format = fmt if out_format is None else out_format
         self.check_out = co if check_out is None else check_out
-        W, H = info.width, info.height
+        reduce = 0 <= top_level < info.levels
+        shift = info.levels - top_level if reduce else 0
+
+        def cd(a):
+            return -(-a // (1 << shift))
+        W, H = cd(info.x0 + info.width) - cd(info.x0), cd(info.y0 + info.height) - cd(info.y0)
+        self.width, self.height = W, H
         self.planes, descs, tables, seg_parts = [], [], [], []
         self.out_planes = []
         if self.out_format == FMT_NONE:
@@ -589,6 +607,7 @@ class CodestreamDecoder:
             self.out = torch.zeros((H, W * PIXEL_BYTES[self.out_format]), dtype=torch.uint8,
                                    device=device)
         seg_base = 0
+        nkept = 0
         for t in range(cs.ntiles):
             x0, y0, w, h, nb, sb = cs.tile_info(t)
             pitch = (w + 63) // 64 * 64
@@ -596,10 +615,13 @@ class CodestreamDecoder:
             self.planes.append(pl)
             ptrs = [pl[c].data_ptr() for c in range(info.ncomp)]
             blocks, segs, desc = cs.tile_blocks(t, ptrs, pitch, seg_base)
+            if reduce and desc.levels != info.levels:
+                raise BmiError("reduced-resolution decode of tiles with different level counts (-5)")
             desc.out_format = self.out_format
             desc.check_out = self.check_out
             desc.r53_mode = r53_mode
-            ox, oy = x0 - info.x0, y0 - info.y0
+            desc.top_level = top_level if reduce else -1
+            ox, oy = cd(x0) - cd(info.x0), cd(y0) - cd(info.y0)
             if self.out is not None:
                 desc.out = self.out.data_ptr()
                 desc.out_pitch_px = W
@@ -610,10 +632,28 @@ class CodestreamDecoder:
                 desc.planes_pitch = W
             descs.append(desc)
             _check_block_table(blocks, nb, ptrs, pitch, h, seg_base, sb)
-            tables.append(bytes(blocks)[:nb * ctypes.sizeof(BmiT1Block)])
+            size = ctypes.sizeof(BmiT1Block)
+            if reduce and desc.levels > 0:
+                # keep the blocks inside the Mallat region of the target
+                # resolution: the LL band and the detail bands of the levels
+                # that are synthesised
+                rw, rh = cd(x0 + w) - cd(x0), cd(y0 + h) - cd(y0)
+                raw = bytes(blocks)
+                keep = []
+                for i in range(nb):
+                    b = blocks[i]
+                    c = next(k for k, p in enumerate(ptrs) if p <= b.out < p + 4 * pitch * h)
+                    y, x = divmod((b.out - ptrs[c]) // 4, pitch)
+                    if x < rw and y < rh:
+                        keep.append(raw[i * size:(i + 1) * size])
+                tables.append(b"".join(keep))
+                nkept += len(keep)
+            else:
+                tables.append(bytes(blocks)[:nb * size])
+                nkept += nb
             seg_parts.append(segs[:sb].tobytes())
             seg_base += sb
-        self.nblocks = info.total_blocks
+        self.nblocks = nkept
         seg_all = b"".join(seg_parts) + b"\xff" * 16
         self.seg_bytes = seg_base
         self.d_segs = torch.from_numpy(np.frombuffer(seg_all, np.uint8).copy()).to(device)

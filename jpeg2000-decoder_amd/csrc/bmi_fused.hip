@@ -662,16 +662,44 @@ __device__ __attribute__((always_inline)) inline void process_item_small(const L
     }
 }
 
+// Thumbnail (a tile decoded with zero synthesis levels, top_level 0): the
+// LL band alone, colour-converted and packed like a final level.  9/7
+// samples are dequantised with the LL step only, (float)twos(w) * step:
+// with the gains folded into the synthesis passes (K / 2/K per direction)
+// every synthesis level has DC gain 1, so this is the scale the full
+// decode gives a flat region.  The reference's GPU thumbnail
+// (cudainterface.cpp:1278-1293, 1496-1510) converts the LL band of a
+// 5/3 tile the same way; one item is 64 pixels of one LL row.
+template <int FILT, int NC, int OUTK, int PL>
+__device__ __attribute__((always_inline)) inline void thumb_item(const LevelJob &__restrict__ J,
+                                                                  int strip, int seg, int lane)
+{
+    using T = typename SampleT<FILT>::T;
+    const int x = J.a0 + strip * 64 + lane;
+    const int y = J.b0 + seg;
+    if (x >= J.a1) return;
+    Emitter<FILT, NC, OUTK, PL> em;
+    em.init(J);
+    T v[NC];
+    const size_t idx = (size_t)seg * (size_t)J.ll_pitch + (size_t)(x - J.a0);
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+        v[c] = (c < J.nc) ? load_sample<FILT>(J.ll[c], idx, J.ll_kind, J.gain[c][0]) : (T)0;
+    em.put(y, x, v);
+}
+
 // Kernel variants: V = (filt == 97) * 32 + sub * 4 + kd.
 // sub: 0 plane (1 comp), 1 merge (1 comp), 2..5 RGB of layout
 // PL_BGR8..PL_BGRA16 (3 comps), 6 merge (2-3 comps), 7 merge (4 comps).
 // kd: coefficient encodings (9/7 planes and RGB only; KD_ANY otherwise).
+// Thumbnail jobs (lev < 0): V = 64 + (filt == 97) * 8 + sub.
 __host__ __device__ inline int variant_of(const LevelJob &J)
 {
     int sub;
     if (J.out_kind == OUT_SCRATCH || J.out_kind == OUT_PLANE) sub = 0;
     else if (J.out_kind == OUT_RGB) sub = 2 + pixel_layout(J.fmt);
     else sub = J.nc == 1 ? 1 : (J.nc <= 3 ? 6 : 7);
+    if (J.lev < 0) return 64 + (J.filt == 97 ? 8 : 0) + sub;
     int kd = KD_ANY;
     if (J.filt == 97 && (sub == 0 || (sub >= 2 && sub <= 5))) {
         const bool llf = J.ll_kind == SRC_FLOAT, detf = J.det_kind == SRC_FLOAT;
@@ -684,6 +712,16 @@ template <int V>
 __device__ __attribute__((always_inline)) inline void run_variant(const LevelJob &__restrict__ J,
                                                                  int strip, int seg, int lane)
 {
+    if constexpr (V >= 64) {
+        constexpr int TF = (V & 8) ? 97 : 53;
+        constexpr int TS = V & 7;
+        if constexpr (TS == 0) thumb_item<TF, 1, K_PLANE, PL_ANY>(J, strip, seg, lane);
+        else if constexpr (TS == 1) thumb_item<TF, 1, K_MERGE, PL_ANY>(J, strip, seg, lane);
+        else if constexpr (TS >= 2 && TS <= 5) thumb_item<TF, 3, K_RGB, TS - 2>(J, strip, seg, lane);
+        else if constexpr (TS == 6) thumb_item<TF, 3, K_MERGE, PL_ANY>(J, strip, seg, lane);
+        else thumb_item<TF, 4, K_MERGE, PL_ANY>(J, strip, seg, lane);
+        return;
+    }
     constexpr int FILT = (V >= 32) ? 97 : 53;
     constexpr int SUB = (V >> 2) & 7;
     constexpr int KD = V & 3;
@@ -698,7 +736,9 @@ __device__ __attribute__((always_inline)) inline void run_variant(const LevelJob
 #define BMI_FOR_EACH_VARIANT(X)                                                             \
     X(0) X(4) X(8) X(12) X(16) X(20) X(24) X(28)                                            \
     X(32) X(33) X(34) X(35) X(36) X(40) X(41) X(42) X(43) X(44) X(45) X(46) X(47) X(48)     \
-    X(49) X(50) X(51) X(52) X(53) X(54) X(55) X(56) X(60)
+    X(49) X(50) X(51) X(52) X(53) X(54) X(55) X(56) X(60)                               \
+    X(64) X(65) X(66) X(67) X(68) X(69) X(70) X(71)                                         \
+    X(72) X(73) X(74) X(75) X(76) X(77) X(78) X(79)
 
 __device__ __attribute__((always_inline)) inline void run_item(
     const LevelJob *__restrict__ jobs, const WorkItem &it, int lane)
@@ -872,7 +912,6 @@ int validate_tile(const BmiTileDesc &t)
     if (t.width < 1 || t.height < 1 || t.width > 32767 || t.height > 32767) return BMI_E_INVALID;
     if (t.x0 < 0 || t.y0 < 0) return BMI_E_INVALID;
     if (t.levels < 1 || t.levels > MAX_DWT_LEVEL) return BMI_E_INVALID;
-    if (t.top_level == 0) return BMI_E_UNSUPPORTED;
     if (t.top_level < -1 || t.top_level > t.levels) return BMI_E_INVALID;
     if (t.ncomp < 1 || t.ncomp > MAX_COMPONENT) return BMI_E_INVALID;
     if (t.method != Ckernels_W5X3 && t.method != Ckernels_W9X7) return BMI_E_INVALID;
@@ -992,6 +1031,8 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
     int max_steps = 0;
     for (int ti = 0; ti < ntiles; ++ti)
         max_steps = std::max(max_steps, tiles[ti].top_level < 0 ? tiles[ti].levels : tiles[ti].top_level);
+    for (int ti = 0; ti < ntiles; ++ti)
+        if (tiles[ti].top_level == 0) max_steps = std::max(max_steps, 1);  // thumbnails run in step 0
     std::vector<std::vector<int>> job_of(ntiles);  // [tile][lev*4+comp] -> job
     for (int ti = 0; ti < ntiles; ++ti) job_of[ti].assign(MAX_DWT_LEVEL * 4, -1);
     const int seg_rows = P->tuning.seg_rows;
@@ -1005,6 +1046,62 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
             const int L = t.levels;
             const int Ls = (t.top_level < 0) ? L : t.top_level;
             const int lev = step;
+            if (Ls == 0 && step == 0) {
+                // thumbnail: the LL band of the coarsest level, no synthesis
+                const int filt = (t.method == Ckernels_W9X7) ? 97 : 53;
+                const int ok = out_kind_for(t);
+                LevelJob J;
+                memset(&J, 0, sizeof(J));
+                J.a0 = ceil_shift(t.x0, L);
+                J.a1 = ceil_shift(t.x0 + t.width, L);
+                J.b0 = ceil_shift(t.y0, L);
+                J.b1 = ceil_shift(t.y0 + t.height, L);
+                J.lw = J.a1 - J.a0;
+                J.lh = J.b1 - J.b0;
+                J.filt = filt;
+                J.lev = -1;
+                J.out_kind = ok;
+                J.ll_kind = (filt == 97) ? (t.coef_kind == BMI_COEF_FLOAT ? SRC_FLOAT : SRC_SIGNMAG)
+                                         : SRC_INT;
+                J.det_kind = J.ll_kind;
+                J.ll_pitch = J.det_pitch = t.coef_pitch;
+                J.nstrips = (J.lw + 63) / 64;
+                J.nsegs = J.lh;
+                J.seg_pairs = 1;
+                J.n_items = J.nstrips * J.nsegs;
+                J.fmt = t.out_format;
+                J.bit_depth = t.bit_depth;
+                J.check_out = t.check_out;
+                J.out = t.out;
+                J.out_pitch_px = t.out_pitch_px;
+                J.out_x = t.out_x;
+                J.out_y = t.out_y;
+                int comps[4], ncomps = 0;
+                needed_comps(t, ok, comps, &ncomps);
+                auto fill = [&](LevelJob &K, int slot, int c) {
+                    K.comp[slot] = c;
+                    K.ll[slot] = K.det[slot] = t.coef[c];
+                    K.gain[slot][0] = (filt == 97) ? t.steps[c][0] : 1.0f;
+                };
+                if (ok == OUT_PLANE) {
+                    for (int i = 0; i < ncomps; ++i) {
+                        LevelJob K = J;
+                        K.nc = 1;
+                        fill(K, 0, comps[i]);
+                        K.dst[0] = t.planes_out[comps[i]];
+                        K.dst_pitch = t.planes_pitch;
+                        P->jobs.push_back(K);
+                        P->job_tile.push_back(ti);
+                    }
+                } else {
+                    J.nc = ncomps;
+                    for (int i = 0; i < 4; ++i) fill(J, i, comps[i < ncomps ? i : 0]);
+                    if (ok == OUT_MERGE && t.check_out >= 0) J.check_out = 0;
+                    P->jobs.push_back(J);
+                    P->job_tile.push_back(ti);
+                }
+                continue;
+            }
             if (lev >= Ls) continue;
             const bool final_level = (lev == Ls - 1);
             const int ok = final_level ? out_kind_for(t) : (int)OUT_SCRATCH;
@@ -1449,7 +1546,7 @@ extern "C" int BMI_batch_plan_rebind(BmiBatchPlan *P, int i, const BmiTileDesc *
             // padded lockstep slots (s >= nc) repeat slot 0's component
             const int c = J.comp[s];
             J.det[s] = t->coef[c];
-            if (J.lev == 0) J.ll[s] = t->coef[c];
+            if (J.lev <= 0) J.ll[s] = t->coef[c];
             if (J.out_kind == OUT_PLANE && s < J.nc) J.dst[s] = t->planes_out[c];
         }
         if (J.out_kind == OUT_RGB || J.out_kind == OUT_MERGE) {

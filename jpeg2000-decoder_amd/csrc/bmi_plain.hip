@@ -643,6 +643,8 @@ extern "C" int BMI_plain_decode(const BmiTileDesc *t, void *temp, size_t temp_by
     if (t->levels < 0 || t->levels > MAX_DWT_LEVEL || t->ncomp < 1 || t->ncomp > 4 ||
         t->width < 1 || t->height < 1)
         return bmi_fail(BMI_E_INVALID, "plain_decode: bad tile");
+    if (t->top_level == 0)
+        return bmi_fail(BMI_E_UNSUPPORTED, "plain_decode: thumbnails (top_level 0) are batch-plan only");
     const int L = t->levels;
     // subband table (sizes as cudainterface.cpp:674-703 from the standard geometry)
     std::vector<SubbandInfo_c> sbs((size_t)t->ncomp * (3 * L + 1));

@@ -341,7 +341,20 @@ int or_idwt97(uint32_t *plane, int pitch, int w, int h, int x0, int y0, int leve
     float *col = (float *)malloc(sizeof(float) * (size_t)h);
     if (!tmp || !col) { free(tmp); free(col); return -3; }
     if (top == 0) {
-        /* no synthesis: the LL is left as is (coefficients) */
+        /* thumbnail (no synthesis): the LL band dequantised with its own
+           step only, (float)twos(w) * step.  With the reference's gains
+           folded into the synthesis passes each level has DC gain 1, so
+           this is the flat-region scale of the full decode.  The
+           reference's GPU thumbnail (cudainterface.cpp:1278-1293,
+           1496-1510) colour-converts the LL band; it defines no 9/7
+           dequantisation for it. */
+        int lw = ((x0 + w + (1 << L) - 1) >> L) - ((x0 + (1 << L) - 1) >> L);
+        int lh = ((y0 + h + (1 << L) - 1) >> L) - ((y0 + (1 << L) - 1) >> L);
+        for (int y = 0; y < lh; ++y)
+            for (int x = 0; x < lw; ++x) {
+                float v = load97(plane[(size_t)y * pitch + x], cf) * steps[0];
+                memcpy(&plane[(size_t)y * pitch + x], &v, 4);
+            }
         free(tmp);
         free(col);
         return 0;

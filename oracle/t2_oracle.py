@@ -639,14 +639,19 @@ def tile_planes(tile, orc):
 
 
 def decode_image(buf: bytes, orc, *, out_format: int, check_out: int = -1, r53_mode: int = 0,
-                 guard_t800: bool = False):
+                 guard_t800: bool = False, toplevel: int = -1):
     """Whole-image oracle decode: T2 (this module), T1 and the reference
     pipeline (oracle.decode_tile) per tile, tiles placed in one packed
     image (rows of width * pixel bytes), or per-component planes for
     out_format 7 (NONE)."""
     P = parse(buf, guard_t800)
     siz = P["siz"]
-    W, H = siz["Xsiz"] - siz["XOsiz"], siz["Ysiz"] - siz["YOsiz"]
+    L0 = P["tiles"][0]["levels"]
+    sh = L0 - toplevel if 0 <= toplevel < L0 else 0
+
+    def cd(a):
+        return -(-a // (1 << sh))
+    W, H = cd(siz["Xsiz"]) - cd(siz["XOsiz"]), cd(siz["Ysiz"]) - cd(siz["YOsiz"])
     depth = siz["comps"][0]["depth"]
     nc = siz["Csiz"]
     if out_format == 7:
@@ -659,9 +664,10 @@ def decode_image(buf: bytes, orc, *, out_format: int, check_out: int = -1, r53_m
         rec, packed = orc.decode_tile(planes, levels=T["levels"], filt=T["filt"],
                                       out_format=out_format, bit_depth=depth, check_out=check_out,
                                       x0=T["tx0"], y0=T["ty0"], steps=steps,
-                                      coef_kind=orc.COEF_SIGNMAG, r53_mode=r53_mode)
-        ox, oy = T["tx0"] - siz["XOsiz"], T["ty0"] - siz["YOsiz"]
-        h, w = T["ty1"] - T["ty0"], T["tx1"] - T["tx0"]
+                                      coef_kind=orc.COEF_SIGNMAG, r53_mode=r53_mode,
+                                      toplevel=toplevel if sh else -1)
+        ox, oy = cd(T["tx0"]) - cd(siz["XOsiz"]), cd(T["ty0"]) - cd(siz["YOsiz"])
+        h, w = cd(T["ty1"]) - cd(T["ty0"]), cd(T["tx1"]) - cd(T["tx0"])
         if out_format == 7:
             for c in range(nc):
                 out[c][oy:oy + h, ox:ox + w] = np.asarray(rec[c]).view(np.int32)

@@ -134,3 +134,25 @@ def test_gpu_own_writer_codestream_equals_t1_path(pkg, orc, workloads):
     _, packed = orc.decode_tile(planes, levels=5, filt="97", out_format=pkg.FMT_24BITS_RGB,
                                 steps=img["steps"], coef_kind=orc.COEF_SIGNMAG)
     assert np.array_equal(got, packed)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["rgb97_tiles_pcrl.j2k", "rgb_tiles_offsets.j2k", "grey_layers_lrcp.j2k",
+                                  "rgb97_256.j2k"])
+@pytest.mark.parametrize("top", [0, 1, 3])
+def test_gpu_codestream_reduced_resolution(pkg, orc, name, top):
+    """Reduced-resolution and thumbnail decodes of whole files (only the
+    code-blocks of the needed bands go through tier 1) equal the oracle's
+    decode at that resolution."""
+    import torch
+    data = (J2K / name).read_bytes()
+    print(f"case {name} top {top}", flush=True)
+    dec = pkg.CodestreamDecoder(data, top_level=top)
+    dec.run(0)
+    torch.cuda.synchronize()
+    got = dec.out.cpu().numpy()
+    assert dec.nblocks < dec.cs.info.total_blocks
+    exp = t2.decode_image(data, orc, out_format=dec.out_format, check_out=dec.check_out, toplevel=top)
+    assert got.shape == exp.shape
+    assert np.array_equal(got, exp), (name, top)
+    dec.close()

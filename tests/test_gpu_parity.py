@@ -99,6 +99,28 @@ def test_reduced_resolution_top_level(pkg, workloads, orc, filt, top):
         assert np.array_equal(got, exp), f"{run.__name__}: {diff_report(got, exp)}"
 
 
+@pytest.mark.parametrize("filt,coef_kind", [("53", 1), ("97", 1), ("97", 0)])
+@pytest.mark.parametrize("fmt,ncomp,check_out", [(2, 3, -1), (0, 3, 1), (6, 1, 0), (3, 4, -1), (7, 3, -1)])
+def test_thumbnail_top_level_0(pkg, workloads, orc, filt, coef_kind, fmt, ncomp, check_out):
+    """top_level 0: the LL band alone (9/7 dequantised with its step),
+    colour-converted and packed like a final level, at odd tile offsets."""
+    w, h, L, x0, y0 = 301, 157, 4, 3, 5
+    planes = workloads.tile_planes(w, h, L, filt, ncomp, seed=9, x0=x0, y0=y0, ll53=(-60, 60),
+                                   ll97=(0.0, 40.0), signmag=(coef_kind == 0))
+    steps = [np.linspace(0.5 + c * 0.1, 2.0, 3 * L + 1).astype(np.float32) for c in range(ncomp)]
+    exp_planes, exp = orc.decode_tile(planes, levels=L, filt=filt, out_format=fmt, toplevel=0,
+                                      x0=x0, y0=y0, steps=steps, coef_kind=coef_kind,
+                                      check_out=check_out)
+    got_planes, got = run_fused(pkg, planes, filt=filt, levels=L, out_format=fmt, top_level=0,
+                                x0=x0, y0=y0, steps=steps, coef_kind=coef_kind, check_out=check_out)
+    if fmt == 7:
+        for g, e in zip(got_planes, exp_planes):
+            assert np.array_equal(np.asarray(g).view(np.uint32), np.asarray(e).view(np.uint32))
+    else:
+        assert got.shape == exp.shape
+        assert np.array_equal(got, exp), diff_report(got, exp)
+
+
 def test_signmag_steps_match_oracle(pkg, workloads, orc):
     """T1 sign-magnitude words with per-band mfAbsStepNorm (the reference's
     BMI_comp_idwt 9/7 contract)."""
