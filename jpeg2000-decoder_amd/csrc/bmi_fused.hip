@@ -870,8 +870,9 @@ struct BmiBatchPlan {
     } tuning;
     LevelJob *d_jobs = nullptr;
     WorkItem *d_items = nullptr;
-    int *d_counters = nullptr;  // [njobs] completion counters, [njobs] = head,
-                                // [njobs+1] = error flag
+    int *d_counters = nullptr;  // [njobs] completion counters; [njobs] queue head,
+                                // [njobs+1] exited waves, [njobs+2] error flag
+                                // (persistent mode)
     void *scratch = nullptr;     // device base of the plan's memory (scratch first)
     size_t scratch_bytes = 0;
     size_t device_bytes = 0;     // scratch + job table + items + counters
@@ -1400,7 +1401,7 @@ static int launch_one(const BmiBatchPlan *P, const BmiBatchPlan::Launch &l, hipS
         BMI_FOR_EACH_VARIANT(BMI_STEP_CASE)
 #undef BMI_STEP_CASE
     default:
-        break;
+        return bmi_fail(BMI_E_INVALID, "fused launch: no kernel for variant %d", l.variant);
     }
     return BMI_OK;
 }
@@ -1431,7 +1432,10 @@ static int plan_enqueue(BmiBatchPlan *P, hipStream_t s, const std::vector<hipStr
             while (j < nl && P->launches[j].step == P->launches[i].step) ++j;
             const int nb = branches ? (int)std::min(branches->size(), j - i) : 0;
             if (nb <= 1) {
-                for (size_t k = i; k < j; ++k) launch_one(P, P->launches[k], s);
+                for (size_t k = i; k < j; ++k) {
+                    const int rc = launch_one(P, P->launches[k], s);
+                    if (rc) return rc;
+                }
             } else {
                 // fork: every branch waits for what s has enqueued so far
                 hipEvent_t fork;
@@ -1443,8 +1447,10 @@ static int plan_enqueue(BmiBatchPlan *P, hipStream_t s, const std::vector<hipStr
                 for (int b = 0; b < nb; ++b)
                     if (hipStreamWaitEvent((*branches)[b], fork, 0) != hipSuccess)
                         return bmi_fail(BMI_E_HIP, "fused launch: fork wait");
-                for (size_t k = i; k < j; ++k)
-                    launch_one(P, P->launches[k], (*branches)[(k - i) % nb]);
+                for (size_t k = i; k < j; ++k) {
+                    const int rc = launch_one(P, P->launches[k], (*branches)[(k - i) % nb]);
+                    if (rc) return rc;
+                }
                 // join: s waits for every branch
                 for (int b = 0; b < nb; ++b) {
                     hipEvent_t join;
