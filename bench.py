@@ -86,7 +86,8 @@ def _device_planes(torch, dev, w, h, filt, ncomp, gen):
 
 def batched_leg(pkg, wl, torch, dev, stream, steps, warmup):
     """256 tiles (sides 64..1024, 3..6 levels, 5/3 / 9/7, 1 or 3
-    components) in one plan: autotuned launch shape, hipGraph replay;
+    components) in one plan: autotuned launch shape, hipGraph replay and
+    direct launches timed (the faster reported);
     beside it the plain path decoding the same tiles one per call."""
     tiles = wl.config3_params(0)
     gen = torch.Generator(device=dev)
@@ -111,16 +112,21 @@ def batched_leg(pkg, wl, torch, dev, stream, steps, warmup):
     best, best_us = pkg.autotune(descs, None, reps=3, stream=sh)
     tune_s = time.perf_counter() - t0
     plan = pkg.BatchPlan(descs, best)
-    for _ in range(warmup):
-        plan.execute_graph(sh)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(steps):
-        plan.execute_graph(sh)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / steps
+    mode_ms = {}
+    for name, fn in (("hipGraph", lambda: plan.execute_graph(sh)),
+                     ("direct launches", lambda: plan.execute(sh))):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        mode_ms[name] = e0.elapsed_time(e1) / steps
+    launch_mode = min(mode_ms, key=mode_ms.get)
+    ms = mode_ms[launch_mode]
     compulsory = comp_samples * 4 + out_bytes
     res = {"tiles": len(tiles), "pixels": pixels, "ms_per_batch": round(ms, 4),
            "Mpixel_s": round(pixels / (ms * 1e-3) / 1e6, 1),
@@ -128,6 +134,8 @@ def batched_leg(pkg, wl, torch, dev, stream, steps, warmup):
            "GBs_compulsory": round(compulsory / (ms * 1e-3) / 1e9, 1),
            "frac_of_peak": round(compulsory / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "work_items": plan.num_items, "launches": plan.num_launches,
+           "launch_mode": launch_mode,
+           "us_per_batch_by_launch_mode": {k: round(v * 1e3, 1) for k, v in mode_ms.items()},
            "tuning": {"seg_rows": best.seg_rows, "waves_per_group": best.waves_per_group,
                       "min_waves_per_cu": best.min_waves_per_cu, "persistent": best.persistent},
            "autotune_s": round(tune_s, 2)}

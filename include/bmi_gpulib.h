@@ -159,8 +159,12 @@ typedef struct BmiTuning {
     int seg_rows;        /* maximum output rows per work item (even) */
     int waves_per_group; /* 1..16 wave64s per workgroup */
     int groups_per_cu;   /* persistent workgroups per CU */
-    int persistent;      /* 1 = one persistent launch with a device work
-                            queue; 0 (default) = one launch per level step */
+    int persistent;      /* 0 (default) = one launch per level step (a step
+                            mixing kernel variants runs them in one
+                            dispatching launch); 1 = one persistent launch
+                            with a device work queue; 2 = every step as one
+                            dispatching launch; 3 = one launch per step and
+                            variant */
     int min_waves_per_cu; /* segment heights shrink until each level step
                              offers this many items per CU (default 4) */
 } BmiTuning;

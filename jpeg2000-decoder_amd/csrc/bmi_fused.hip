@@ -847,6 +847,27 @@ __global__ void __launch_bounds__(256) fused_step_kernel(const LevelJob *__restr
     run_variant<V>(J, local % J.nstrips, local / J.nstrips, lane);
 }
 
+// level-step mode with all kernel variants of a step in one launch (each
+// wave dispatches on its item's variant): one launch per step instead of
+// one per step and variant, at the register budget of the largest variant
+__global__ void __launch_bounds__(256) fused_step_any_kernel(const LevelJob *__restrict__ jobs,
+                                                             const WorkItem *__restrict__ items,
+                                                             int first, int count, int remap)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    int g = blockIdx.x;
+    if (remap) {
+        const int G = gridDim.x, x = g & 7, j = g >> 3, q = G >> 3, r = G & 7;
+        g = x * q + min(x, r) + j;
+    }
+    const int i = g * (blockDim.x >> 6) + wave;
+    if (i >= count) return;
+    const int job = __builtin_amdgcn_readfirstlane(items[first + i].job);
+    const int local = __builtin_amdgcn_readfirstlane(items[first + i].local);
+    run_item(jobs, WorkItem{job, local}, lane);
+}
+
 }  // namespace bmi
 
 // ---------------------------------------------------------------------------
@@ -1272,6 +1293,26 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                 P->launches.push_back({first, (int)P->items.size() - first, order[i].first, step});
                 i = k;
             }
+            // A step that needs several kernel variants (a batch of mixed
+            // filters / components / outputs) becomes one launch of the
+            // dispatching kernel (variant -1) unless the tuning asks for one
+            // launch per variant (3); mode 2 merges every step.  One launch
+            // per step beats concurrent per-variant launches for mixed
+            // batches; a single-variant step keeps its specialised kernel.
+            {
+                int n0 = -1, nv = 0;
+                for (size_t q = 0; q < P->launches.size(); ++q)
+                    if (P->launches[q].step == step) {
+                        if (n0 < 0) n0 = (int)q;
+                        ++nv;
+                    }
+                const int mode = P->tuning.persistent;
+                if (n0 >= 0 && ((mode == 0 && nv > 1) || mode == 2)) {
+                    const int f = P->launches[n0].first;
+                    P->launches.resize(n0);
+                    P->launches.push_back({f, (int)P->items.size() - f, -1, step});
+                }
+            }
         }
         P->step_count[step] = (int)P->items.size() - P->step_first[step];
     }
@@ -1392,6 +1433,11 @@ static int launch_one(const BmiBatchPlan *P, const BmiBatchPlan::Launch &l, hipS
     const int wpg = P->tuning.waves_per_group;
     const int xcd_remap = 1;
     const dim3 grid((l.count + wpg - 1) / wpg), block(64 * wpg);
+    if (l.variant < 0) {
+        hipLaunchKernelGGL(fused_step_any_kernel, grid, block, 0, s, P->d_jobs, P->d_items, l.first,
+                           l.count, xcd_remap);
+        return BMI_OK;
+    }
     switch (l.variant) {
 #define BMI_STEP_CASE(V)                                                                        \
     case V:                                                                                    \
@@ -1419,7 +1465,7 @@ static int plan_enqueue(BmiBatchPlan *P, hipStream_t s, const std::vector<hipStr
     if (n == 0) return BMI_OK;
     const int wpg = P->tuning.waves_per_group;
     (void)hipGetLastError();  // report this plan's launches, not an earlier error
-    if (P->tuning.persistent) {
+    if (P->tuning.persistent == 1) {
         const int max_groups = P->num_cus * P->tuning.groups_per_cu;
         const int need_groups = (n + wpg - 1) / wpg;
         const int groups = std::max(1, std::min(max_groups, need_groups));
@@ -1473,12 +1519,12 @@ static int plan_enqueue(BmiBatchPlan *P, hipStream_t s, const std::vector<hipStr
 extern "C" int BMI_batch_plan_num_launches(const BmiBatchPlan *P)
 {
     if (!P) return 0;
-    return P->tuning.persistent ? 1 : (int)P->launches.size();
+    return P->tuning.persistent == 1 ? 1 : (int)P->launches.size();
 }
 
 extern "C" int BMI_batch_plan_execute_launches(BmiBatchPlan *P, int first, int count, StreamId stream)
 {
-    if (!P || P->tuning.persistent || first < 0 || count < 0 ||
+    if (!P || P->tuning.persistent == 1 || first < 0 || count < 0 ||
         first + count > (int)P->launches.size())
         return bmi_fail(BMI_E_INVALID, "batch_plan_execute_launches: bad range");
     auto saved = P->launches;

@@ -283,7 +283,7 @@ void BmiServer::run()
                 ++i;
                 continue;
             }
-            if (cfg.tuning.persistent && e == hipSuccess && inflight[i].plan &&
+            if (cfg.tuning.persistent == 1 && e == hipSuccess && inflight[i].plan &&
                 BMI_batch_plan_error_flag(inflight[i].plan) != 0)
                 e = hipErrorLaunchFailure;
             stream_busy[inflight[i].stream] = false;

@@ -64,6 +64,18 @@ def test_batch_step_mode_matches_oracle(pkg, batch):
     plan.close()
 
 
+@pytest.mark.parametrize("mode", [2, 3])
+def test_batch_launch_grouping_matches_oracle(pkg, batch, mode):
+    """Every step as one dispatching launch (2) and one launch per step and
+    kernel variant (3) decode as the default grouping does."""
+    _clear(batch)
+    plan = pkg.BatchPlan([d for d, _ in batch["cases"]], pkg.BmiTuning(32, 4, 4, mode, 0))
+    assert plan.num_launches >= 1
+    plan.execute(0)
+    _check_all(batch, f"launch grouping {mode}")
+    plan.close()
+
+
 def test_batch_persistent_queue_matches_oracle(pkg, batch):
     _clear(batch)
     plan = pkg.BatchPlan([d for d, _ in batch["cases"]], pkg.BmiTuning(32, 4, 4, 1, 0))
