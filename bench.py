@@ -315,7 +315,8 @@ def codestream_leg(pkg, wl, torch, dev, stream, steps, warmup, img, ref_out):
     dec = pkg.CodestreamDecoder(data, out_format=pkg.FMT_24BITS_RGB)
     sh = stream.cuda_stream
     _, _, w, h, nb, sb = dec.cs.tile_info(0)
-    pinned = torch.empty(sb + 16, dtype=torch.uint8, pin_memory=True)
+    # the decoder's device copy holds the tile's share padded to 16 bytes
+    pinned = torch.full((dec.d_segs.numel(),), 0xFF, dtype=torch.uint8, pin_memory=True)
     pn = pinned.numpy()
     ptrs = [dec.planes[0][c].data_ptr() for c in range(3)]
     pitch = dec.planes[0].shape[2]

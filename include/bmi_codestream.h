@@ -14,10 +14,12 @@
  *     unsigned, not sub-sampled (the reference's GPU path, which has no
  *     sub-sampled or signed formats);
  *   - COD/COC: LRCP, RLCP, RPCL, PCRL, CPRL; any number of layers; default
- *     or explicit precincts; SOP / EPH markers; code-block style 0 or
- *     predictable termination (the reference asserts on bypass,
- *     bmi_block_decoder.cpp:792-797, and ignores reset / termall / causal /
- *     segmentation symbols); code-blocks up to 64x64;
+ *     or explicit precincts; SOP / EPH markers; every code-block style of
+ *     T.800 Table A.19 (bypass, reset, termination on each pass, vertically
+ *     causal contexts, predictable termination, segmentation symbols: the
+ *     reference decodes style 0 only and asserts on bypass,
+ *     bmi_block_decoder.cpp:792-797), with one segment length per codeword
+ *     segment in the packet headers (B.10.7.2); code-blocks up to 64x64;
  *   - QCD/QCC: no quantisation (5/3), scalar derived or expounded (9/7);
  *   - COM, TLM, PLM, PLT, CRG are skipped; RGN, POC, PPM, PPT are rejected
  *     with BMI_E_UNSUPPORTED.
@@ -81,11 +83,17 @@ int BMI_codestream_tile_info(const BmiCodestream *cs, int t, int *x0, int *y0, i
                          the device coefficient planes planes[c] (pitch
                          `plane_pitch` 32-bit elements, tile-sized, Mallat
                          layout: the layout the IDWT reads);
-     segs                the code-block segments (host buffer of
+     segs                the code-block data (host buffer of
                          seg_capacity >= seg_bytes + 16 bytes: the GPU
-                         decoder reads up to 16 bytes past a segment), each
-                         block's layers concatenated, at blocks[i].data_offset
-                         + seg_base;
+                         decoder reads up to 16 bytes past a segment): first
+                         the segment-length tables of blocks whose code-block
+                         style splits their passes into several codeword
+                         segments (int32, at blocks[i].seg_offset - seg_base,
+                         16-byte aligned in segs), then each block's codeword
+                         segments, every one gathered over the layers, at
+                         blocks[i].data_offset - seg_base.  seg_base is the
+                         offset of segs[0] in the `data` buffer later given to
+                         BMI_t1_decode_blocks (a multiple of 16);
      desc                the tile descriptor: coefficient planes, geometry,
                          levels, filter, bit depth, steps, colour transform.
                          Output fields (out, out_format, out_pitch_px, out_x,

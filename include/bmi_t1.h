@@ -44,7 +44,28 @@ typedef struct BmiT1Block {
     int kind;      /* BMI_T1_WORDS / BMI_T1_INT */
     void *out;     /* device address of the block's top-left output word */
     int out_pitch; /* elements between output rows */
+    /* Code-block style (COD/COC SPcod, T.800 Table A.19; 0 in every stream
+       the reference decodes).  With BMI_T1_BYPASS or BMI_T1_TERMALL the
+       `len` bytes hold `nseg` codeword segments back to back, segment k
+       being L[k] bytes, where L is an int32 array at byte offset
+       `seg_offset` (a multiple of 4) of `data`; nseg must be the number of
+       segments the block's num_pass passes open (BMI_t1_num_segments).
+       Otherwise the data is one segment and nseg / seg_offset are unused.
+       The reference asserts on these styles (bmi_block_decoder.cpp:792-797,
+       818-822); they are decoded here as T.800 Annex D defines them. */
+    int style;
+    int nseg;
+    unsigned long long seg_offset;
 } BmiT1Block;
+
+/* code-block style bits (T.800 Table A.19) */
+#define BMI_T1_BYPASS 0x01   /* raw significance / refinement passes from pass 10 on (D.6) */
+#define BMI_T1_RESET 0x02    /* context states reset on every pass boundary */
+#define BMI_T1_TERMALL 0x04  /* every pass is a terminated codeword segment */
+#define BMI_T1_VSC 0x08      /* vertically stripe-causal contexts (D.7) */
+#define BMI_T1_PTERM 0x10    /* predictable termination (no effect on decoding) */
+#define BMI_T1_SEGSYM 0x20   /* segmentation symbol after every cleanup pass (D.5) */
+#define BMI_T1_STYLES 0x3F
 
 /* Decode `nblocks` code-blocks described by the DEVICE array `blocks`, whose
    segments are in the DEVICE buffer `data`; `data` must stay readable for
@@ -56,8 +77,15 @@ typedef struct BmiT1Block {
 int BMI_t1_decode_blocks(const BmiT1Block *blocks, int nblocks, const void *data,
                          StreamId stream);
 /* Host-side validation of a block table before upload: returns BMI_OK or
-   the first error (index in *bad when not NULL). */
+   the first error (index in *bad when not NULL).  Segment lengths live in
+   device memory and are not read here: the decoder clamps each segment to
+   the block's `len` bytes, so a wrong table decodes wrong words but reads
+   nothing outside the block's data. */
 int BMI_t1_check_blocks(const BmiT1Block *blocks, int nblocks, int *bad);
+/* Codeword segments opened by the first num_pass coding passes of a block
+   of code-block style `style` (1 for num_pass > 0 without BMI_T1_BYPASS /
+   BMI_T1_TERMALL). */
+int BMI_t1_num_segments(int num_pass, int style);
 
 #ifdef __cplusplus
 }

@@ -155,7 +155,14 @@ class BmiT1Block(ctypes.Structure):
         ("kind", ctypes.c_int),
         ("out", ctypes.c_void_p),
         ("out_pitch", ctypes.c_int),
+        ("style", ctypes.c_int),
+        ("nseg", ctypes.c_int),
+        ("seg_offset", ctypes.c_ulonglong),
     ]
+
+
+# code-block style bits (include/bmi_t1.h, ITU-T T.800 Table A.19)
+T1_BYPASS, T1_RESET, T1_TERMALL, T1_VSC, T1_PTERM, T1_SEGSYM = 0x01, 0x02, 0x04, 0x08, 0x10, 0x20
 
 
 # BMI_codestream_open flags (include/bmi_codestream.h)
@@ -254,6 +261,7 @@ def lib() -> ctypes.CDLL:
     L.BMI_t1_decode_blocks.argtypes = [vp, i, vp, vp]
     L.BMI_t1_decode_blocks.restype = i
     L.BMI_t1_check_blocks.argtypes = [ctypes.POINTER(BmiT1Block), i, ctypes.POINTER(i)]
+    L.BMI_t1_num_segments.argtypes = [i, i]
     L.BMI_t1_check_blocks.restype = i
     L.BMI_plain_decode.argtypes = [ctypes.POINTER(BmiTileDesc), vp, ctypes.c_size_t, vp]
     L.BMI_plain_decode.restype = i
@@ -552,10 +560,12 @@ def default_format(info: BmiCodestreamInfo):
 
 
 def _check_block_table(blocks, nb: int, plane_ptrs, pitch: int, rows: int, seg_base: int,
-                       seg_bytes: int) -> None:
+                       seg_bytes: int, segs=None) -> None:
     """Host-side bounds check of a tile's block table before it reaches
-    the GPU: every block's output rectangle lies inside its tile plane and
-    every segment inside the tile's share of the segment buffer."""
+    the GPU: every block's output rectangle lies inside its tile plane,
+    every segment inside the tile's share of the segment buffer, and
+    (given the host copy `segs` of that share) every segment-length table
+    inside it, aligned, and adding up to the block's bytes."""
     if nb:
         t1_check_blocks([blocks[i] for i in range(nb)])
     for i in range(nb):
@@ -568,6 +578,15 @@ def _check_block_table(blocks, nb: int, plane_ptrs, pitch: int, rows: int, seg_b
             raise BmiError(f"block {i}: output rectangle outside the tile plane")
         if b.data_offset < seg_base or b.data_offset + b.len > seg_base + seg_bytes:
             raise BmiError(f"block {i}: segment outside the tile's data")
+        if b.style & (T1_BYPASS | T1_TERMALL) and b.nseg:
+            t0 = b.seg_offset - seg_base
+            if t0 < 0 or t0 % 4 or t0 + 4 * b.nseg > seg_bytes:
+                raise BmiError(f"block {i}: segment-length table outside the tile's data")
+            if segs is not None:
+                import numpy as np
+                lens = np.frombuffer(bytes(segs[t0:t0 + 4 * b.nseg]), dtype=np.int32)
+                if (lens < 0).any() or int(lens.sum()) != b.len:
+                    raise BmiError(f"block {i}: segment lengths do not add up to the block's bytes")
 
 
 class CodestreamDecoder:
@@ -631,7 +650,7 @@ class CodestreamDecoder:
                     desc.planes_out[c] = self.out_planes[c][oy:, ox:].data_ptr()
                 desc.planes_pitch = W
             descs.append(desc)
-            _check_block_table(blocks, nb, ptrs, pitch, h, seg_base, sb)
+            _check_block_table(blocks, nb, ptrs, pitch, h, seg_base, sb, segs)
             size = ctypes.sizeof(BmiT1Block)
             if reduce and desc.levels > 0:
                 # keep the blocks inside the Mallat region of the target
@@ -651,8 +670,11 @@ class CodestreamDecoder:
             else:
                 tables.append(bytes(blocks)[:nb * size])
                 nkept += nb
-            seg_parts.append(segs[:sb].tobytes())
-            seg_base += sb
+            # each tile's share starts 16-byte aligned (its segment-length
+            # tables are read as aligned words)
+            pad = -sb % 16
+            seg_parts.append(segs[:sb].tobytes() + b"\xff" * pad)
+            seg_base += sb + pad
         self.nblocks = nkept
         seg_all = b"".join(seg_parts) + b"\xff" * 16
         self.seg_bytes = seg_base

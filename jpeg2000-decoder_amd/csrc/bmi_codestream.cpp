@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "bmi_codestream.h"
+#include "bmi_t1_style.h"
 
 int bmi_fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 
@@ -224,16 +225,20 @@ int read_num_passes(BitReader &br)
     return 37 + br.bits(7);
 }
 
+// bytes of codeword segment `seg` of a code-block, found in one packet
 struct Chunk {
     uint32_t off, len;
+    int seg;
 };
 
 struct Block {
-    int comp, band, x, y, w, h, orient, numbps;
-    int missing = 0, npass = 0, lblock = 3, pending = 0;
+    int comp, band, x, y, w, h, orient, numbps, style = 0;
+    int missing = 0, npass = 0, lblock = 3;
     bool seen = false;
-    std::vector<Chunk> chunks;
+    std::vector<Chunk> chunks;   // in stream order; segments never interleave
+    std::vector<Chunk> pending;  // lengths read from the current packet header
     size_t bytes = 0;
+    int nseg() const { return bmi_style_segmented(style) ? BMI_t1_num_segments(npass, style) : 1; }
 };
 
 struct Precinct {
@@ -268,7 +273,8 @@ struct Tile {
     std::vector<TileComp> comps;
     std::vector<Block> blocks;
     float steps[MAX_COMPONENT][3 * MAX_DWT_LEVEL + 1];
-    size_t seg_bytes = 0;
+    size_t seg_bytes = 0;     // segment-length tables, then the block data
+    size_t table_bytes = 0;   // of which the tables (4 bytes per segment)
 };
 
 }  // namespace
@@ -572,7 +578,8 @@ void build_tile(BmiCodestream *S, int t, Tile &T)
         if (T.have_qcd) tc.q = T.qcd;
         for (auto &e : T.qcc) if (e.first == c) tc.q = e.second;
         const CodingStyle &cs = tc.cod;
-        if (cs.cbstyle & ~0x10) fail(BMI_E_UNSUPPORTED, "code-block style other than 0 / predictable termination");
+        if (cs.cbstyle & ~BMI_T1_STYLES)
+            fail(BMI_E_UNSUPPORTED, "code-block style " + std::to_string(cs.cbstyle));
         if (cs.xcb > 6 || cs.ycb > 6) fail(BMI_E_UNSUPPORTED, "code-blocks over 64 samples a side");
         if (cs.transform > 1) fail(BMI_E_UNSUPPORTED, "wavelet transform other than 9/7 / 5/3");
         if (cs.levels > MAX_DWT_LEVEL) fail(BMI_E_UNSUPPORTED, "more than 6 decomposition levels");
@@ -623,6 +630,7 @@ void build_tile(BmiCodestream *S, int t, Tile &T)
                         const int x0 = std::max((gx0 + i) << xcb, bx0), y0 = std::max((gy0 + j) << ycb, by0);
                         const int x1 = std::min((gx0 + i + 1) << xcb, bx1), y1 = std::min((gy0 + j + 1) << ycb, by1);
                         b.comp = c; b.band = bid; b.orient = o; b.numbps = numbps;
+                        b.style = tc.cod.cbstyle;
                         b.x = x0 - bx0 + llw * xo; b.y = y0 - by0 + llh * yo;
                         b.w = x1 - x0; b.h = y1 - y0;
                         T.blocks.push_back(std::move(b));
@@ -734,15 +742,27 @@ void read_packet(Tile &T, BitReader &br, const PacketId &id, bool eph_flag)
                     cb.seen = true;
                 }
                 const int np = read_num_passes(br);
+                const int p0 = cb.npass;
                 cb.npass += np;
                 if (cb.npass > 128) fail(BMI_E_INVALID, "more than 128 coding passes");
                 while (br.bit()) {
                     if (++cb.lblock > 32) fail(BMI_E_INVALID, "Lblock over 32");
                 }
-                const int nbits = cb.lblock + floor_log2((unsigned)np);
-                if (nbits > 32) fail(BMI_E_INVALID, "segment length field over 32 bits");
-                cb.pending = (int)br.bits(nbits);
-                if (cb.pending < 0) fail(BMI_E_INVALID, "segment length");
+                // one length per codeword segment the new passes reach
+                // (T.800 B.10.7.2): Lblock + floor(log2(its passes here)) bits
+                int seg = -1;
+                for (int i = 0; i <= p0; ++i) seg += bmi_pass_new_segment(i, cb.style) ? 1 : 0;
+                for (int i = p0; i < p0 + np;) {
+                    if (i > p0 && bmi_pass_new_segment(i, cb.style)) ++seg;
+                    int m = 1;
+                    while (i + m < p0 + np && !bmi_pass_new_segment(i + m, cb.style)) ++m;
+                    const int nbits = cb.lblock + floor_log2((unsigned)m);
+                    if (nbits > 32) fail(BMI_E_INVALID, "segment length field over 32 bits");
+                    const int64_t ln = (int64_t)br.bits(nbits);
+                    if (ln < 0 || ln > 0x7FFFFFFF) fail(BMI_E_INVALID, "segment length");
+                    cb.pending.push_back({0, (uint32_t)ln, seg});
+                    i += m;
+                }
             }
         }
     }
@@ -753,12 +773,13 @@ void read_packet(Tile &T, BitReader &br, const PacketId &id, bool eph_flag)
         if (B.prec.empty()) continue;
         for (int bi : B.prec[id.prec].blocks) {
             Block &cb = T.blocks[bi];
-            if (!cb.pending) continue;
-            if ((size_t)cb.pending > n - br.pos) fail(BMI_E_INVALID, "code-block data past the end of the tile");
-            cb.chunks.push_back({(uint32_t)br.pos, (uint32_t)cb.pending});
-            cb.bytes += cb.pending;
-            br.pos += cb.pending;
-            cb.pending = 0;
+            for (const Chunk &c : cb.pending) {
+                if ((size_t)c.len > n - br.pos) fail(BMI_E_INVALID, "code-block data past the end of the tile");
+                cb.chunks.push_back({(uint32_t)br.pos, c.len, c.seg});
+                cb.bytes += c.len;
+                br.pos += c.len;
+            }
+            cb.pending.clear();
         }
     }
 }
@@ -774,8 +795,13 @@ void decode_tile(BmiCodestream *S, int t)
         if (br.pos >= T.data.size()) break;   // truncated after a packet: the rest are absent
         read_packet(T, br, id, eph);
     }
-    T.seg_bytes = 0;
-    for (const Block &b : T.blocks) T.seg_bytes += b.bytes;
+    size_t data = 0, nsegs = 0;
+    for (const Block &b : T.blocks) {
+        data += b.bytes;
+        if (bmi_style_segmented(b.style)) nsegs += (size_t)b.nseg();
+    }
+    T.table_bytes = (4 * nsegs + 15) & ~(size_t)15;
+    T.seg_bytes = T.table_bytes + data;
 }
 
 }  // namespace
@@ -854,20 +880,33 @@ extern "C" int BMI_codestream_tile_blocks(const BmiCodestream *S, int t, void *c
     const Tile &T = S->tiles[t];
     const int w = T.tx1 - T.tx0, h = T.ty1 - T.ty0;
     if (plane_pitch < w) return bmi_fail(BMI_E_INVALID, "plane pitch %d below tile width %d", plane_pitch, w);
+    if (seg_base & 15) return bmi_fail(BMI_E_INVALID, "seg_base %llu not a multiple of 16", seg_base);
     if (blocks && (!planes || (segs == nullptr && T.seg_bytes)))
         return bmi_fail(BMI_E_INVALID, "BMI_codestream_tile_blocks: planes / segs missing");
     if (blocks) {
         if (seg_capacity < T.seg_bytes + 16) return bmi_fail(BMI_E_INVALID, "segment buffer below seg_bytes + 16");
         for (int c = 0; c < S->Csiz; ++c)
             if (!planes[c]) return bmi_fail(BMI_E_INVALID, "plane %d missing", c);
+        // layout: the segment-length tables of segmented blocks, then
+        // every block's data (its segments in order), then 16 bytes of 0xFF
         uint8_t *dst = static_cast<uint8_t *>(segs);
-        size_t off = 0;
+        int32_t *table = reinterpret_cast<int32_t *>(dst);
+        size_t off = T.table_bytes, ntab = 0;
+        memset(dst, 0, T.table_bytes);
         const int reversible = T.comps[0].cod.transform == 1;
         for (size_t i = 0; i < T.blocks.size(); ++i) {
             const Block &b = T.blocks[i];
             BmiT1Block &o = blocks[i];
             o.data_offset = seg_base + off;
             o.len = (int)b.bytes;
+            o.style = b.style;
+            o.nseg = b.nseg();
+            o.seg_offset = 0;
+            if (bmi_style_segmented(b.style)) {
+                o.seg_offset = seg_base + 4 * ntab;
+                for (const Chunk &ch : b.chunks) table[ntab + ch.seg] += (int32_t)ch.len;
+                ntab += (size_t)o.nseg;
+            }
             for (const Chunk &ch : b.chunks) {
                 memcpy(dst + off, T.data.data() + ch.off, ch.len);
                 off += ch.len;

@@ -22,13 +22,17 @@
 // 589-853) and its passes (:10-586), the MQ decoder
 // (decoder/codec/src/bmi_mq_decoder.cpp:186-355) and dec_cb_stripe's copy
 // (decoder/codec/src/dec_t1_comp.cpp:283-426); the procedures are written
-// from ITU-T T.800 C.3 and D.
+// from ITU-T T.800 C.3 and D.  Code-block styles other than 0 (bypass with raw
+// passes, context reset, termination on every pass, vertically causal
+// contexts, segmentation symbols: T.800 Table A.19, D.5-D.7), which the
+// reference asserts on, run in a second instantiation of the kernel.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "bmi_device.h"
 #include "bmi_internal.h"
 #include "bmi_t1.h"
+#include "bmi_t1_style.h"
 
 namespace bmi {
 namespace {
@@ -230,25 +234,64 @@ struct Mq {
             CT = 8;
         }
     }
-    __device__ __forceinline__ void init(const uint8_t *sg, int n, uint32_t lane, int orient)
+    // per code-block: context tables, Table C.2 and the initial states
+    __device__ __forceinline__ void init_block(uint32_t lane, int orient)
     {
         zl = build_lut(lane, orient, false);
         sl = build_lut(lane, orient, true);
+        qtab = lane < 47 ? kMqTable[lane] : 0u;
+        reset_states(lane);
+    }
+    // all contexts index 0 / MPS 0, except zero coding 0 (index 4), run
+    // (index 3) and uniform (index 46)
+    __device__ __forceinline__ void reset_states(uint32_t lane)
+    {
+        ctx = kMqTable[lane == 0 ? 4 : (lane == CX_RL ? 3 : (lane == CX_UNI ? 46 : 0))];
+    }
+    __device__ __forceinline__ void open(const uint8_t *sg, int n)
+    {
         len = n;
         seg = sg;
         buf = 0;
         nbuf = 0;
         bp = 0;
         fill();
-        qtab = lane < 47 ? kMqTable[lane] : 0u;
-        // all contexts index 0 / MPS 0, except zero coding 0 (index 4),
-        // run (index 3) and uniform (index 46)
-        ctx = kMqTable[lane == 0 ? 4 : (lane == CX_RL ? 3 : (lane == CX_UNI ? 46 : 0))];
+    }
+    // INITDEC on a codeword segment (the context states carry over)
+    __device__ __forceinline__ void start(const uint8_t *sg, int n)
+    {
+        open(sg, n);
         C = byte_at(0) << 16;
         bytein();
         C <<= 7;
         CT -= 7;
         A = 0x8000;
+    }
+    // Raw segment (bypass passes, T.800 D.6): bits most significant first,
+    // seven bits in a byte that follows 0xFF.  bp is the next unread byte;
+    // 0xFF followed by a byte above 0x8F (a marker, or the 0xFF bytes past
+    // the segment) is not consumed and reads as 1 bits.  C holds the
+    // current byte and CT its unread bits.
+    __device__ __forceinline__ void start_raw(const uint8_t *sg, int n)
+    {
+        open(sg, n);
+        C = 0;
+        CT = 0;
+    }
+    __device__ __forceinline__ uint32_t raw_bit()
+    {
+        if (CT == 0) {
+            const uint32_t nb = byte_at(0);
+            if (C == 0xFF && nb > 0x8F) {
+                CT = 8;
+            } else {
+                CT = (C == 0xFF) ? 7 : 8;
+                C = nb;
+                advance();
+            }
+        }
+        --CT;
+        return (C >> CT) & 1;
     }
     // RENORMD: shift A and C until bit 15 of A is set, reading a byte each
     // time CT runs out (several shifts at once between reads)
@@ -340,13 +383,15 @@ __device__ __forceinline__ void or_row(uint32_t &lo, uint32_t &hi, int r, uint32
 // and the pending bits of the stripes above / below
 template <int R>
 __device__ __forceinline__ void significant(Stripe &S, uint32_t &lo, uint32_t &hi, int x, int w,
-                                            uint32_t neg)
+                                            uint32_t neg, bool vsc)
 {
     or_row(lo, hi, R, S_SIG | (neg ? S_NEG : 0));
     const uint32_t vn = S_N | (neg ? S_NNEG : 0), vs = S_S | (neg ? S_SNEG : 0);
     if (R > 0) or_row(lo, hi, R - 1, vs);
     if (R < 3) or_row(lo, hi, R + 1, vn);
-    if (R == 0) S.pend_prev = wrl(rdl(S.pend_prev, x) | (vs << 16), x, S.pend_prev);
+    // with vertically causal contexts the last row of the stripe above
+    // does not see this stripe
+    if (R == 0 && !vsc) S.pend_prev = wrl(rdl(S.pend_prev, x) | (vs << 16), x, S.pend_prev);
     if (R == 3) S.pend_next = wrl(rdl(S.pend_next, x) | vn, x, S.pend_next);
     // the columns to the left and right
 #pragma unroll
@@ -362,31 +407,39 @@ __device__ __forceinline__ void significant(Stripe &S, uint32_t &lo, uint32_t &h
         if (R < 3) or_row(clo, chi, R + 1, down);
         if (clo) S.lo = wrl(rdl(S.lo, c) | clo, c, S.lo);
         if (chi) S.hi = wrl(rdl(S.hi, c) | chi, c, S.hi);
-        if (R == 0) S.pend_prev = wrl(rdl(S.pend_prev, c) | (up << 16), c, S.pend_prev);
+        if (R == 0 && !vsc) S.pend_prev = wrl(rdl(S.pend_prev, c) | (up << 16), c, S.pend_prev);
         if (R == 3) S.pend_next = wrl(rdl(S.pend_next, c) | down, c, S.pend_next);
     }
 }
 
 // significance propagation of row R of column x
+// (raw: the significance bit and then the sign bit itself, uncoded)
 template <int R>
 __device__ __forceinline__ void sigprop_row(Mq &m, Stripe &S, uint32_t &lo, uint32_t &hi, int x,
                                             int w, int rows, int orient, uint32_t one_half,
-                                            uint32_t lane)
+                                            uint32_t lane, bool raw, bool vsc)
 {
     if (R >= rows) return;
     const uint32_t s = row_state<R>(lo, hi);
     if ((s & S_SIG) || !(s & S_NB)) return;
     or_row(lo, hi, R, S_VISIT);
-    if (!m.decode(zc_lut(m.zl, s), lane)) return;
-    const uint32_t sc = sc_lut(m.sl, s);
-    const uint32_t neg = m.decode(sc & 0xFF, lane) ^ (sc >> 8);
-    significant<R>(S, lo, hi, x, w, neg);
+    uint32_t neg;
+    if (raw) {
+        if (!m.raw_bit()) return;
+        neg = m.raw_bit();
+    } else {
+        if (!m.decode(zc_lut(m.zl, s), lane)) return;
+        const uint32_t sc = sc_lut(m.sl, s);
+        neg = m.decode(sc & 0xFF, lane) ^ (sc >> 8);
+    }
+    significant<R>(S, lo, hi, x, w, neg, vsc);
     vrow<R>(S) = wrl((neg << 31) + one_half, x, vrow<R>(S));
 }
 
 template <int R>
 __device__ __forceinline__ void magref_row(Mq &m, Stripe &S, uint32_t lo, uint32_t hi, int x,
-                                           int rows, int plane, uint32_t half, uint32_t lane)
+                                           int rows, int plane, uint32_t half, uint32_t lane,
+                                           bool raw)
 {
     if (R >= rows) return;
     const uint32_t s = row_state<R>(lo, hi);
@@ -394,7 +447,7 @@ __device__ __forceinline__ void magref_row(Mq &m, Stripe &S, uint32_t lo, uint32
     const uint32_t v = rdl(vrow<R>(S), x);
     const uint32_t mag = (v & 0x7FFFFFFFu) >> plane;
     const uint32_t cx = mag < 4 ? CX_MR + ((s & S_NB) ? 1 : 0) : CX_MR + 2;
-    const uint32_t bit = m.decode(cx, lane);
+    const uint32_t bit = raw ? m.raw_bit() : m.decode(cx, lane);
     vrow<R>(S) = wrl((v ^ ((1u - bit) << plane)) | half, x, vrow<R>(S));
 }
 
@@ -402,7 +455,7 @@ __device__ __forceinline__ void magref_row(Mq &m, Stripe &S, uint32_t lo, uint32
 template <int R>
 __device__ __forceinline__ void cleanup_row(Mq &m, Stripe &S, uint32_t &lo, uint32_t &hi, int x,
                                             int w, int rows, int orient, uint32_t one_half,
-                                            int first, uint32_t lane)
+                                            int first, uint32_t lane, bool vsc)
 {
     if (R >= rows || R < first) return;
     const uint32_t s = row_state<R>(lo, hi);
@@ -410,18 +463,18 @@ __device__ __forceinline__ void cleanup_row(Mq &m, Stripe &S, uint32_t &lo, uint
     if (!m.decode(zc_lut(m.zl, s), lane)) return;
     const uint32_t sc = sc_lut(m.sl, s);
     const uint32_t neg = m.decode(sc & 0xFF, lane) ^ (sc >> 8);
-    significant<R>(S, lo, hi, x, w, neg);
+    significant<R>(S, lo, hi, x, w, neg, vsc);
     vrow<R>(S) = wrl((neg << 31) + one_half, x, vrow<R>(S));
 }
 
 template <int R>
 __device__ __forceinline__ void run_hit(Mq &m, Stripe &S, uint32_t &lo, uint32_t &hi, int x, int w,
-                                        uint32_t one_half, uint32_t lane)
+                                        uint32_t one_half, uint32_t lane, bool vsc)
 {
     const uint32_t s = row_state<R>(lo, hi);
     const uint32_t sc = sc_lut(m.sl, s);
     const uint32_t neg = m.decode(sc & 0xFF, lane) ^ (sc >> 8);
-    significant<R>(S, lo, hi, x, w, neg);
+    significant<R>(S, lo, hi, x, w, neg, vsc);
     vrow<R>(S) = wrl((neg << 31) + one_half, x, vrow<R>(S));
 }
 
@@ -432,6 +485,12 @@ __device__ __forceinline__ void rows_mask(int rows, uint32_t bits, uint32_t &mlo
     mhi = (rows > 2 ? bits : 0) | (rows > 3 ? bits << 16 : 0);
 }
 
+// Two instantiations, launched one after the other over the same table:
+// STYLED false decodes the blocks of the code-block style the reference
+// decodes (0, or predictable termination), STYLED true those with any
+// other switch of T.800 Table A.19; each skips the other's blocks.  Kept
+// apart so that the style machinery costs the common kernel no registers.
+template <bool STYLED>
 __global__ void __launch_bounds__(64 * kWavesPerGroup)
     t1_decode_kernel(const BmiT1Block *__restrict__ blocks, int nblocks,
                      const uint8_t *__restrict__ data)
@@ -456,6 +515,12 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup)
     const uintptr_t outp = ((uintptr_t)uni((uint32_t)((uintptr_t)B->out >> 32)) << 32) |
                            uni((uint32_t)(uintptr_t)B->out);
     const int out_pitch = (int)uni(B->out_pitch);
+    const int block_style = (int)uni(B->style) & BMI_T1_STYLES;
+    if (((block_style & ~BMI_T1_PTERM) != 0) != STYLED) return;
+    const int nseg = bmi_style_segmented(block_style) ? (int)uni(B->nseg) : 1;
+    const uint64_t segl_off = ((uint64_t)uni((uint32_t)(B->seg_offset >> 32)) << 32) |
+                              uni((uint32_t)B->seg_offset);
+    const auto *segl = (const __attribute__((address_space(4))) int *)(data + segl_off);
     const int nstripes = (h + 3) >> 2;
     const uint64_t colmask = (w == 64) ? ~0ull : ((1ull << w) - 1);
 
@@ -483,10 +548,16 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup)
         if (plane < 0) plane = p_max;
         if (plane == 0) plane = 1;
     }
-    if (num_passes > 0) {
+    // the passes (with STYLED false every style switch folds away)
+    {
+        const int style = STYLED ? block_style : 0;
+        const bool segmented = STYLED && bmi_style_segmented(style);
+        const bool vsc = STYLED && (style & BMI_T1_VSC);
         Mq m;
-        m.init(data + off, len, lane, orient);
+        if (num_passes > 0) m.init_block(lane, orient);
         int type = 2;
+        int seg = 0, seg_pos = 0;
+        bool raw = false;
         for (int pi = 0; pi < num_passes; ++pi, ++type) {
             if (type == 3) {
                 type = 0;
@@ -495,6 +566,20 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup)
             // passes below bit-plane 0 are not decoded (a negative shift
             // count, undefined in the reference)
             if (plane < 0) break;
+            if (bmi_pass_new_segment(pi, style)) {
+                // next codeword segment, clamped to the block's bytes (a
+                // missing or oversized length reads nothing beyond them)
+                int n = segmented ? (seg < nseg ? (int)uni((uint32_t)segl[seg]) : 0) : len;
+                n = max(0, min(n, len - seg_pos));
+                raw = bmi_pass_raw(pi, style);
+                if (raw)
+                    m.start_raw(data + off + seg_pos, n);
+                else
+                    m.start(data + off + seg_pos, n);
+                seg_pos += n;
+                ++seg;
+            }
+            if ((style & BMI_T1_RESET) && pi > 0) m.reset_states(lane);
             const uint32_t one_half = (plane >= 0 && plane < 31) ? (1u << plane) + ((1u << plane) >> 1) : 0u;
             const uint32_t half = (plane >= 1 && plane < 32) ? (1u << plane) >> 1 : 0u;
             Stripe S;
@@ -549,18 +634,18 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup)
                     uint32_t lo = rdl(S.lo, x), hi = rdl(S.hi, x);
                     const uint32_t sig0 = (lo | (hi << 1)) & (S_SIG | (S_SIG << 16) | (S_SIG << 1) | (S_SIG << 17));
                     if (type == 0) {
-                        sigprop_row<0>(m, S, lo, hi, x, w, rows, orient, one_half, lane);
-                        sigprop_row<1>(m, S, lo, hi, x, w, rows, orient, one_half, lane);
-                        sigprop_row<2>(m, S, lo, hi, x, w, rows, orient, one_half, lane);
-                        sigprop_row<3>(m, S, lo, hi, x, w, rows, orient, one_half, lane);
+                        sigprop_row<0>(m, S, lo, hi, x, w, rows, orient, one_half, lane, raw, vsc);
+                        sigprop_row<1>(m, S, lo, hi, x, w, rows, orient, one_half, lane, raw, vsc);
+                        sigprop_row<2>(m, S, lo, hi, x, w, rows, orient, one_half, lane, raw, vsc);
+                        sigprop_row<3>(m, S, lo, hi, x, w, rows, orient, one_half, lane, raw, vsc);
                         // a sample that became significant makes column x+1 a candidate
                         const uint32_t sig1 = (lo | (hi << 1)) & (S_SIG | (S_SIG << 16) | (S_SIG << 1) | (S_SIG << 17));
                         if (x + 1 < w && sig1 != sig0) todo |= 1ull << (x + 1);
                     } else if (type == 1) {
-                        magref_row<0>(m, S, lo, hi, x, rows, plane, half, lane);
-                        magref_row<1>(m, S, lo, hi, x, rows, plane, half, lane);
-                        magref_row<2>(m, S, lo, hi, x, rows, plane, half, lane);
-                        magref_row<3>(m, S, lo, hi, x, rows, plane, half, lane);
+                        magref_row<0>(m, S, lo, hi, x, rows, plane, half, lane, raw);
+                        magref_row<1>(m, S, lo, hi, x, rows, plane, half, lane, raw);
+                        magref_row<2>(m, S, lo, hi, x, rows, plane, half, lane, raw);
+                        magref_row<3>(m, S, lo, hi, x, rows, plane, half, lane, raw);
                         continue;  // refinement leaves the state words unchanged
                     } else {
                         int first = 0;
@@ -571,18 +656,18 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup)
                                 first = (int)m.decode(CX_UNI, lane) << 1;
                                 first |= (int)m.decode(CX_UNI, lane);
                                 switch (first) {
-                                case 0: run_hit<0>(m, S, lo, hi, x, w, one_half, lane); break;
-                                case 1: run_hit<1>(m, S, lo, hi, x, w, one_half, lane); break;
-                                case 2: run_hit<2>(m, S, lo, hi, x, w, one_half, lane); break;
-                                default: run_hit<3>(m, S, lo, hi, x, w, one_half, lane); break;
+                                case 0: run_hit<0>(m, S, lo, hi, x, w, one_half, lane, vsc); break;
+                                case 1: run_hit<1>(m, S, lo, hi, x, w, one_half, lane, vsc); break;
+                                case 2: run_hit<2>(m, S, lo, hi, x, w, one_half, lane, vsc); break;
+                                default: run_hit<3>(m, S, lo, hi, x, w, one_half, lane, vsc); break;
                                 }
                                 ++first;
                             }
                         }
-                        cleanup_row<0>(m, S, lo, hi, x, w, rows, orient, one_half, first, lane);
-                        cleanup_row<1>(m, S, lo, hi, x, w, rows, orient, one_half, first, lane);
-                        cleanup_row<2>(m, S, lo, hi, x, w, rows, orient, one_half, first, lane);
-                        cleanup_row<3>(m, S, lo, hi, x, w, rows, orient, one_half, first, lane);
+                        cleanup_row<0>(m, S, lo, hi, x, w, rows, orient, one_half, first, lane, vsc);
+                        cleanup_row<1>(m, S, lo, hi, x, w, rows, orient, one_half, first, lane, vsc);
+                        cleanup_row<2>(m, S, lo, hi, x, w, rows, orient, one_half, first, lane, vsc);
+                        cleanup_row<3>(m, S, lo, hi, x, w, rows, orient, one_half, first, lane, vsc);
                     }
                     S.lo = wrl(lo, x, S.lo);
                     S.hi = wrl(hi, x, S.hi);
@@ -603,6 +688,10 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup)
                 vstore(4 * s + 3, S.v3);
                 if (s > 0) L.st[s - 1][lane][1] |= S.pend_prev;
             }
+            // segmentation symbol (D.5): four uniform-context decisions
+            // after each cleanup pass
+            if (type == 2 && (style & BMI_T1_SEGSYM))
+                for (int k = 0; k < 4; ++k) (void)m.decode(CX_UNI, lane);
         }
     }
 
@@ -632,7 +721,9 @@ extern "C" int BMI_t1_check_blocks(const BmiT1Block *blocks, int nblocks, int *b
         int rc = BMI_OK;
         if (b.width < 1 || b.height < 1 || b.len < 0 || b.num_pass < 0 || b.missing < 0 ||
             b.orient < 0 || b.orient > 3 || (b.kind != BMI_T1_WORDS && b.kind != BMI_T1_INT) ||
-            !b.out || b.out_pitch < b.width)
+            !b.out || b.out_pitch < b.width || (b.style & ~BMI_T1_STYLES) ||
+            (bmi_style_segmented(b.style) &&
+             (b.nseg != BMI_t1_num_segments(b.num_pass, b.style) || (b.seg_offset & 3))))
             rc = BMI_E_INVALID;
         else if (b.width > kMaxSide || b.height > kMaxSide)
             rc = BMI_E_UNSUPPORTED;
@@ -646,6 +737,13 @@ extern "C" int BMI_t1_check_blocks(const BmiT1Block *blocks, int nblocks, int *b
     return BMI_OK;
 }
 
+extern "C" int BMI_t1_num_segments(int num_pass, int style)
+{
+    int n = 0;
+    for (int i = 0; i < num_pass; ++i) n += bmi_pass_new_segment(i, style) ? 1 : 0;
+    return n;
+}
+
 extern "C" int BMI_t1_decode_blocks(const BmiT1Block *blocks, int nblocks, const void *data,
                                     StreamId stream)
 {
@@ -656,7 +754,9 @@ extern "C" int BMI_t1_decode_blocks(const BmiT1Block *blocks, int nblocks, const
     // the check below must see this launch, not an error left by an
     // earlier, unrelated runtime call of the caller
     (void)hipGetLastError();
-    hipLaunchKernelGGL(t1_decode_kernel, grid, block, 0, (hipStream_t)stream, blocks, nblocks,
+    hipLaunchKernelGGL(t1_decode_kernel<false>, grid, block, 0, (hipStream_t)stream, blocks, nblocks,
+                       (const uint8_t *)data);
+    hipLaunchKernelGGL(t1_decode_kernel<true>, grid, block, 0, (hipStream_t)stream, blocks, nblocks,
                        (const uint8_t *)data);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return bmi_fail(BMI_E_HIP, "t1 launch: %s", hipGetErrorString(e));

@@ -9,10 +9,15 @@
  *              magnitude refinement, cleanup with run mode), stripes of four
  *              rows, context tables D.1 (zero coding), D.3 (sign coding),
  *              D.4 (magnitude refinement).
- * Options are the ones the reference block decoder supports
+ * bmi_t1_encode_block writes what the reference block decoder supports
  * (decoder/codec/src/bmi_block_decoder.cpp:589-853): one codeword segment
- * per code-block (no arithmetic-coder bypass, no termination on every
- * pass), no segmentation symbols, no vertically causal contexts, one layer.
+ * per code-block, code-block style 0.  bmi_t1_encode_block_ex adds the
+ * code-block style switches of T.800 Table A.19 that the reference asserts
+ * on (:792-797, 818-822): selective arithmetic coding bypass (raw
+ * significance and refinement passes from the 11th pass on, D.6), context
+ * reset on every pass boundary, termination of every pass, vertically
+ * causal context formation (D.7) and segmentation symbols (D.5); it
+ * returns the byte length of every codeword segment.
  *
  * This file produces inputs; it decodes nothing.
  */
@@ -129,7 +134,8 @@ static void mq_encode(MqEnc *e, int cx, int d)
     }
 }
 
-static void mq_init(MqEnc *e, uint8_t *buf, long cap)
+/* INITENC (C.2.8) on a new codeword segment; the context states are kept */
+static void mq_start(MqEnc *e, uint8_t *buf, long cap)
 {
     e->A = 0x8000;
     e->C = 0;
@@ -139,7 +145,6 @@ static void mq_init(MqEnc *e, uint8_t *buf, long cap)
     e->cap = cap;
     e->overflow = 0;
     buf[-1] = 0;
-    mq_reset_contexts(e);
 }
 
 /* FLUSH (C.2.9): SETBITS, two BYTEOUTs; a final 0xFF byte is not emitted */
@@ -155,6 +160,58 @@ static long mq_flush(MqEnc *e)
     long len = e->bp + 1;
     if (len > 0 && e->buf[len - 1] == 0xFF) --len;
     return len;
+}
+
+/* ---- raw (bypass) segments (D.6) ----------------------------------------
+ * Bits are packed most significant first; a byte that follows 0xFF carries
+ * seven bits (its top bit is a stuffed 0).  Termination pads the last byte
+ * with 0 bits and drops a final 0xFF (a decoder reads 1 bits past the end
+ * of a segment, which is what that byte held). */
+typedef struct {
+    uint8_t *buf;
+    long n, cap;
+    uint32_t acc;
+    int nbits, room, overflow;
+} RawEnc;
+
+static void raw_start(RawEnc *r, uint8_t *buf, long cap)
+{
+    r->buf = buf;
+    r->n = 0;
+    r->cap = cap;
+    r->acc = 0;
+    r->nbits = 0;
+    r->room = 8;
+    r->overflow = 0;
+}
+
+static void raw_emit(RawEnc *r, uint32_t byte)
+{
+    if (r->n >= r->cap) {
+        r->overflow = 1;
+        return;
+    }
+    r->buf[r->n++] = (uint8_t)byte;
+    r->room = (byte == 0xFF) ? 7 : 8;
+}
+
+static void raw_put(RawEnc *r, int bit)
+{
+    r->acc = (r->acc << 1) | (uint32_t)(bit & 1);
+    if (++r->nbits == r->room) {
+        raw_emit(r, r->acc);
+        r->acc = 0;
+        r->nbits = 0;
+    }
+}
+
+static long raw_flush(RawEnc *r)
+{
+    if (r->nbits) raw_emit(r, r->acc << (r->room - r->nbits));
+    r->acc = 0;
+    r->nbits = 0;
+    if (r->n > 0 && r->buf[r->n - 1] == 0xFF) --r->n;
+    return r->n;
 }
 
 /* ---- contexts (T.800 Tables D.1, D.3, D.4) ---------------------------- */
@@ -212,20 +269,52 @@ static int any_sig_neighbour(const uint8_t *f, int stride)
 
 /* ---- the coder ----------------------------------------------------------- */
 
+/* code-block style switches (T.800 Table A.19) */
+enum { CB_BYPASS = 0x01, CB_RESET = 0x02, CB_TERMALL = 0x04, CB_VSC = 0x08, CB_PTERM = 0x10,
+       CB_SEGSYM = 0x20 };
+
+/* Coding pass i of a code-block (0 = the first cleanup pass, then
+   significance, refinement, cleanup per bit-plane: type (i + 2) % 3).
+   With bypass, passes 10, 11, 13, 14, ... (significance and refinement
+   from the fifth coded bit-plane on) are raw; a codeword segment ends at
+   every pass (termall), or with bypass after pass 9, after each raw pair
+   and after each cleanup pass from pass 12 on. */
+static int pass_is_raw(int i, int style) { return (style & CB_BYPASS) && i >= 10 && (i % 3) != 0; }
+static int pass_starts_segment(int i, int style)
+{
+    if (i == 0 || (style & CB_TERMALL)) return 1;
+    return (style & CB_BYPASS) && i >= 10 && (i % 3) != 2;
+}
+
+/* the 3x3 neighbourhood of f as the context rules see it: with vertically
+   causal contexts the row below a stripe's last row reads as insignificant */
+static const uint8_t *nbhood(const uint8_t *f, int stride, int causal, uint8_t g[9])
+{
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx)
+            g[(dy + 1) * 3 + dx + 1] = (causal && dy == 1) ? 0 : f[dy * stride + dx];
+    return &g[4];
+}
+
 /*
  * coef   w*h quantisation indices, two's complement, row pitch w
  * mb     magnitude bit-planes of the band (numbps): |q| < 2^mb required
  * orient 0 LL, 1 HL, 2 LH, 3 HH
  * pass_limit  0 = every pass, else at most this many passes
- * out/cap     output buffer; *len bytes written
+ * style  code-block style bits (CB_*)
+ * out/cap     output buffer; *len bytes written (all segments in order)
  * *num_passes, *missing   passes coded and all-zero leading bit-planes
- * Returns 0, -1 on bad arguments or magnitude overflow, -2 if cap is short.
+ * seg_len, max_segs, nseg   byte length of each codeword segment
+ * Returns 0, -1 on bad arguments or magnitude overflow, -2 if cap or
+ * max_segs is short.
  */
-int bmi_t1_encode_block(const int32_t *coef, int w, int h, int mb, int orient, int pass_limit,
-                        uint8_t *out, long cap, long *len, int *num_passes, int *missing)
+int bmi_t1_encode_block_ex(const int32_t *coef, int w, int h, int mb, int orient, int pass_limit,
+                           int style, uint8_t *out, long cap, long *len, int *num_passes,
+                           int *missing, long *seg_len, int max_segs, int *nseg)
 {
     if (!coef || w < 1 || h < 1 || w > 1024 || h > 1024 || mb < 0 || mb > 30 || orient < 0 ||
-        orient > 3 || !out || cap < 4 || !len || !num_passes || !missing)
+        orient > 3 || !out || cap < 4 || !len || !num_passes || !missing || !seg_len ||
+        max_segs < 1 || !nseg || (style & ~0x3F))
         return -1;
     uint32_t maxmag = 0;
     for (long i = 0; i < (long)w * h; ++i) {
@@ -239,11 +328,13 @@ int bmi_t1_encode_block(const int32_t *coef, int w, int h, int mb, int orient, i
     *missing = mb - K;
     *len = 0;
     *num_passes = 0;
+    *nseg = 0;
     if (K == 0) return 0;
     int total = 3 * K - 2;
     if (pass_limit > 0 && pass_limit < total) total = pass_limit;
 
     const int stride = w + 2;
+    const int vsc = (style & CB_VSC) != 0;
     uint8_t *flags = (uint8_t *)calloc((size_t)stride * (h + 2), 1);
     uint32_t *mag = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)w * h);
     uint8_t *tmp = (uint8_t *)malloc((size_t)cap + 1);
@@ -260,10 +351,35 @@ int bmi_t1_encode_block(const int32_t *coef, int w, int h, int mb, int orient, i
             if (v < 0) flags[(y + 1) * stride + x + 1] |= F_NEG;
         }
     MqEnc e;
-    mq_init(&e, tmp + 1, cap);
+    RawEnc r;
+    mq_reset_contexts(&e);
+    long used = 0;  /* bytes of finished segments in out */
+    int segs = 0, rc = 0, raw = 0;
+    /* end the open segment: terminate its coder and append its bytes */
+    #define END_SEGMENT()                                                              \
+        do {                                                                           \
+            const long n_ = raw ? raw_flush(&r) : mq_flush(&e);                        \
+            if ((raw ? r.overflow : e.overflow) || used + n_ > cap || segs >= max_segs) \
+                rc = -2;                                                               \
+            else {                                                                     \
+                memcpy(out + used, tmp + 1, (size_t)n_);                               \
+                used += n_;                                                            \
+                seg_len[segs++] = n_;                                                  \
+            }                                                                          \
+        } while (0)
+
     int passes = 0;
-    for (int p = K - 1; p >= 0 && passes < total; --p) {
-        for (int type = (p == K - 1) ? 2 : 0; type < 3 && passes < total; ++type, ++passes) {
+    for (int p = K - 1; p >= 0 && passes < total && rc == 0; --p) {
+        for (int type = (p == K - 1) ? 2 : 0; type < 3 && passes < total && rc == 0; ++type, ++passes) {
+            if (pass_starts_segment(passes, style)) {
+                if (passes > 0) END_SEGMENT();
+                raw = pass_is_raw(passes, style);
+                if (raw)
+                    raw_start(&r, tmp + 1, cap);
+                else
+                    mq_start(&e, tmp + 1, cap);
+            }
+            if ((style & CB_RESET) && passes > 0) mq_reset_contexts(&e);
             for (int y0 = 0; y0 < h; y0 += 4) {
                 const int rows = (h - y0) < 4 ? (h - y0) : 4;
                 for (int x = 0; x < w; ++x) {
@@ -272,15 +388,18 @@ int bmi_t1_encode_block(const int32_t *coef, int w, int h, int mb, int orient, i
                         /* run mode: four insignificant, unvisited samples with
                            insignificant neighbourhoods */
                         int run = 1;
-                        for (int r = 0; r < 4 && run; ++r) {
-                            const uint8_t *f = &flags[(y0 + r + 1) * stride + x + 1];
-                            if ((*f & (F_SIG | F_VISIT)) || any_sig_neighbour(f, stride)) run = 0;
+                        for (int k = 0; k < 4 && run; ++k) {
+                            uint8_t g[9];
+                            const uint8_t *f = &flags[(y0 + k + 1) * stride + x + 1];
+                            if ((*f & (F_SIG | F_VISIT)) ||
+                                any_sig_neighbour(nbhood(f, stride, vsc && k == 3, g), 3))
+                                run = 0;
                         }
                         if (run) {
                             int first = -1;
-                            for (int r = 0; r < 4; ++r)
-                                if ((mag[(long)(y0 + r) * w + x] >> p) & 1) {
-                                    first = r;
+                            for (int k = 0; k < 4; ++k)
+                                if ((mag[(long)(y0 + k) * w + x] >> p) & 1) {
+                                    first = k;
                                     break;
                                 }
                             if (first < 0) {
@@ -291,8 +410,9 @@ int bmi_t1_encode_block(const int32_t *coef, int w, int h, int mb, int orient, i
                             mq_encode(&e, CX_UNI, first >> 1);
                             mq_encode(&e, CX_UNI, first & 1);
                             uint8_t *f = &flags[(y0 + first + 1) * stride + x + 1];
+                            uint8_t g[9];
                             int cx, xr;
-                            sc_context(f, stride, &cx, &xr);
+                            sc_context(nbhood(f, stride, vsc && first == 3, g), 3, &cx, &xr);
                             mq_encode(&e, cx, ((*f & F_NEG) ? 1 : 0) ^ xr);
                             *f |= F_SIG;
                             y = y0 + first + 1;
@@ -300,48 +420,75 @@ int bmi_t1_encode_block(const int32_t *coef, int w, int h, int mb, int orient, i
                     }
                     for (; y < y0 + rows; ++y) {
                         uint8_t *f = &flags[(y + 1) * stride + x + 1];
+                        uint8_t g[9];
+                        const uint8_t *n = nbhood(f, stride, vsc && (y & 3) == 3, g);
                         const int bit = (mag[(long)y * w + x] >> p) & 1;
                         if (type == 0) {
-                            if ((*f & F_SIG) || !any_sig_neighbour(f, stride)) continue;
-                            mq_encode(&e, zc_context(f, stride, orient), bit);
+                            if ((*f & F_SIG) || !any_sig_neighbour(n, 3)) continue;
                             *f |= F_VISIT;
+                            if (raw) {
+                                raw_put(&r, bit);
+                                if (bit) {
+                                    raw_put(&r, (*f & F_NEG) ? 1 : 0);
+                                    *f |= F_SIG;
+                                }
+                                continue;
+                            }
+                            mq_encode(&e, zc_context(n, 3, orient), bit);
                         } else if (type == 1) {
                             if (!(*f & F_SIG) || (*f & F_VISIT)) continue;
-                            const int cx = (*f & F_REFINED) ? CX_MR + 2
-                                                            : CX_MR + any_sig_neighbour(f, stride);
-                            mq_encode(&e, cx, bit);
+                            if (raw)
+                                raw_put(&r, bit);
+                            else
+                                mq_encode(&e, (*f & F_REFINED) ? CX_MR + 2 : CX_MR + any_sig_neighbour(n, 3),
+                                          bit);
                             *f |= F_REFINED;
                             continue;
                         } else {
                             if (*f & (F_SIG | F_VISIT)) continue;
-                            mq_encode(&e, zc_context(f, stride, orient), bit);
+                            mq_encode(&e, zc_context(n, 3, orient), bit);
                         }
                         if (bit) {
                             int cx, xr;
-                            sc_context(f, stride, &cx, &xr);
+                            sc_context(n, 3, &cx, &xr);
                             mq_encode(&e, cx, ((*f & F_NEG) ? 1 : 0) ^ xr);
                             *f |= F_SIG;
                         }
                     }
                 }
             }
-            if (type == 2)
+            if (type == 2) {
+                if (style & CB_SEGSYM) { /* 1010 in the uniform context */
+                    mq_encode(&e, CX_UNI, 1);
+                    mq_encode(&e, CX_UNI, 0);
+                    mq_encode(&e, CX_UNI, 1);
+                    mq_encode(&e, CX_UNI, 0);
+                }
                 for (long i = 0; i < (long)stride * (h + 2); ++i) flags[i] &= (uint8_t)~F_VISIT;
+            }
         }
     }
-    const long n = mq_flush(&e);
-    int rc = 0;
-    if (e.overflow || n > cap) {
-        rc = -2;
-    } else {
-        memcpy(out, tmp + 1, (size_t)n);
-        *len = n;
+    if (rc == 0) END_SEGMENT();
+    #undef END_SEGMENT
+    if (rc == 0) {
+        *len = used;
         *num_passes = passes;
+        *nseg = segs;
     }
     free(flags);
     free(mag);
     free(tmp);
     return rc;
+}
+
+/* style 0: one codeword segment (the form the reference decodes) */
+int bmi_t1_encode_block(const int32_t *coef, int w, int h, int mb, int orient, int pass_limit,
+                        uint8_t *out, long cap, long *len, int *num_passes, int *missing)
+{
+    long seg[1];
+    int nseg;
+    return bmi_t1_encode_block_ex(coef, w, h, mb, orient, pass_limit, 0, out, cap, len, num_passes,
+                                  missing, seg, 1, &nseg);
 }
 
 /* the encoder's Table C.2, for the test that compares it with the standard */

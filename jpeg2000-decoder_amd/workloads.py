@@ -197,6 +197,10 @@ def _t1enc():
                                           ctypes.POINTER(ctypes.c_long), ctypes.POINTER(i),
                                           ctypes.POINTER(i)]
         L.bmi_t1_encode_block.restype = i
+        L.bmi_t1_encode_block_ex.argtypes = [P, i, i, i, i, i, i, P, ctypes.c_long,
+                                             ctypes.POINTER(ctypes.c_long), ctypes.POINTER(i),
+                                             ctypes.POINTER(i), P, i, ctypes.POINTER(i)]
+        L.bmi_t1_encode_block_ex.restype = i
         L.bmi_t1_encoder_tables.argtypes = [P, P, P, P]
         L.bmi_t1_encoder_zc.argtypes = [P, i]
         L.bmi_t1_encoder_sc.argtypes = [P]
@@ -204,21 +208,30 @@ def _t1enc():
     return _T1ENC
 
 
-def t1_encode_block(q: np.ndarray, mb: int, orient: int, pass_limit: int = 0):
+def t1_encode_block(q: np.ndarray, mb: int, orient: int, pass_limit: int = 0, style: int = 0,
+                    segments: bool = False):
     """Encode one code-block of quantisation indices q (h, w) with mb
-    magnitude bit-planes -> (bytes, num_passes, missing)."""
+    magnitude bit-planes and code-block style `style` (ITU-T T.800 Table
+    A.19 bits) -> (bytes, num_passes, missing), plus the byte length of
+    every codeword segment when `segments`."""
     import ctypes
     a = np.ascontiguousarray(q, dtype=np.int32)
     h, w = a.shape
-    cap = 64 + 4 * w * h * max(1, mb) // 3 + 4 * w * h
+    cap = 64 + 4 * w * h * max(1, mb) // 3 + 4 * w * h + 8 * 3 * max(1, mb)
     out = np.zeros(cap, dtype=np.uint8)
-    n, npass, miss = ctypes.c_long(), ctypes.c_int(), ctypes.c_int()
-    rc = _t1enc().bmi_t1_encode_block(a.ctypes.data_as(ctypes.c_void_p), w, h, mb, orient,
-                                      pass_limit, out.ctypes.data_as(ctypes.c_void_p), cap,
-                                      ctypes.byref(n), ctypes.byref(npass), ctypes.byref(miss))
+    segs = np.zeros(3 * 32, dtype=np.int64)
+    n, npass, miss, nseg = ctypes.c_long(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    rc = _t1enc().bmi_t1_encode_block_ex(a.ctypes.data_as(ctypes.c_void_p), w, h, mb, orient,
+                                         pass_limit, style, out.ctypes.data_as(ctypes.c_void_p),
+                                         cap, ctypes.byref(n), ctypes.byref(npass),
+                                         ctypes.byref(miss), segs.ctypes.data_as(ctypes.c_void_p),
+                                         len(segs), ctypes.byref(nseg))
     if rc != 0:
         raise ValueError(f"t1 encode failed ({rc}): magnitude >= 2^{mb} or buffer too small")
-    return out[:n.value].tobytes(), npass.value, miss.value
+    data = out[:n.value].tobytes()
+    if segments:
+        return data, npass.value, miss.value, [int(x) for x in segs[:nseg.value]]
+    return data, npass.value, miss.value
 
 
 def band_orient(band_id: int) -> int:
@@ -292,6 +305,98 @@ def t1_image(w: int, h: int, levels: int, ncomp: int, seed: int = 0, cb: int = 6
         off += len(data)
     return dict(q=q, blocks=blocks, data=b"".join(chunks) + b"\xff" * 16, steps=steps,
                 filt=filt, levels=levels, w=w, h=h, ncomp=ncomp)
+
+
+def _fdwt53_axis0(x: np.ndarray) -> np.ndarray:
+    """One forward reversible 5/3 decomposition (ITU-T T.800 F.4.8.2,
+    signal starting at an even index, symmetric extension) along axis 0;
+    returns the low samples followed by the high samples."""
+    n = x.shape[0]
+    if n == 1:
+        return x.copy()
+    ev, od = x[0::2], x[1::2]
+    right = np.concatenate([ev[1:], ev[-1:] if n % 2 == 0 else ev[-2:-1]], axis=0)[:od.shape[0]]
+    d = od - np.floor_divide(ev[:od.shape[0]] + right, 2)
+    dl = np.concatenate([d[:1], d], axis=0)[:ev.shape[0]]           # Y(2k-1), Y(-1) = Y(1)
+    dr = np.concatenate([d, d[-1:]], axis=0)[:ev.shape[0]]          # Y(2k+1), Y(N) = Y(N-2)
+    s = ev + np.floor_divide(dl + dr + 2, 4)
+    return np.concatenate([s, d], axis=0)
+
+
+def forward_53(plane: np.ndarray, levels: int) -> np.ndarray:
+    """Forward 5/3 DWT of a tile at the origin into Mallat layout: per
+    level the vertical decomposition, then the horizontal one (T.800
+    2D_SD, F.4.2), repeated on the LL quadrant."""
+    a = np.asarray(plane, dtype=np.int64).copy()
+    h, w = a.shape
+    for _ in range(levels):
+        if h == 0 or w == 0:
+            break
+        sub = a[:h, :w]
+        sub = _fdwt53_axis0(sub)
+        sub = _fdwt53_axis0(sub.T).T
+        a[:h, :w] = sub
+        h, w = (h + 1) // 2, (w + 1) // 2
+    return a
+
+
+def forward_rct(rgb: np.ndarray) -> np.ndarray:
+    """Reversible colour transform (T.800 G.2) of (..., 3) level-shifted
+    samples -> (Y0, Y1, Y2) = (floor((R+2G+B)/4), B-G, R-G)."""
+    r, g, b = (rgb[..., i].astype(np.int64) for i in range(3))
+    return np.stack([np.floor_divide(r + 2 * g + b, 4), b - g, r - g], axis=-1)
+
+
+def lossless_image(pixels: np.ndarray, levels: int, *, bit_depth: int = 8, cb: int = 64,
+                   style: int = 0, mct: bool | None = None, guard: int = 2,
+                   threads: int = 8) -> dict:
+    """A lossless (reversible 5/3) encode of an image with the project's own
+    tier-1 encoder: level shift, RCT for 3+ components (when `mct`),
+    forward 5/3, code-blocks of cb x cb coded with code-block style
+    `style`.  Band exponents are bit_depth + log2 of the band gain (LL 0,
+    HL / LH 1, HH 2), guard bits `guard`, so a band has
+    guard + exponent - 1 magnitude bit-planes.  Returns the dict of
+    t1_image (filt "53") plus seg_lens per block, style, mct, guard and the
+    band exponents, ready for t1_codestream."""
+    from concurrent.futures import ThreadPoolExecutor
+    px = np.asarray(pixels)
+    if px.ndim == 2:
+        px = px[..., None]
+    h, w, nc = px.shape
+    if mct is None:
+        mct = nc >= 3
+    v = px.astype(np.int64) - (1 << (bit_depth - 1))
+    if mct:
+        v = np.concatenate([forward_rct(v[..., :3]), v[..., 3:]], axis=-1)
+    q = np.stack([forward_53(v[..., c], levels) for c in range(nc)]).astype(np.int32)
+    rects = band_rects(w, h, 0, 0, levels)
+    gain = {0: 0, 1: 1, 2: 1, 3: 2}
+    expn = {bid: bit_depth + gain[band_orient(bid)] for (bid, *_r) in rects}
+    jobs = []
+    for c in range(nc):
+        for (bid, dl, bx, by, bw, bh) in rects:
+            for y0 in range(0, bh, cb):
+                for x0 in range(0, bw, cb):
+                    jobs.append((c, bid, bx + x0, by + y0, min(cb, bw - x0), min(cb, bh - y0)))
+
+    def enc(j):
+        c, bid, x, y, bw, bh = j
+        return t1_encode_block(q[c, y:y + bh, x:x + bw], guard + expn[bid] - 1, band_orient(bid),
+                               style=style, segments=True)
+
+    with ThreadPoolExecutor(threads) as ex:
+        enc_out = list(ex.map(enc, jobs))
+    blocks, chunks, off = [], [], 0
+    for (c, bid, x, y, bw, bh), (data, npass, miss, segs) in zip(jobs, enc_out):
+        blocks.append(dict(comp=c, band=bid, x=x, y=y, w=bw, h=bh, orient=band_orient(bid),
+                           numbps=guard + expn[bid] - 1, missing=miss, num_pass=npass, offset=off,
+                           len=len(data), seg_lens=segs))
+        chunks.append(data)
+        off += len(data)
+    return dict(q=q, blocks=blocks, data=b"".join(chunks) + b"\xff" * 16,
+                steps=np.ones((nc, 3 * levels + 1), np.float32), filt="53", levels=levels, w=w,
+                h=h, ncomp=nc, style=style, mct=bool(mct), guard=guard, expn=expn,
+                bit_depth=bit_depth)
 
 
 # A JPEG 2000 codestream around the code-blocks of t1_image: one tile, one
@@ -378,66 +483,119 @@ def _pass_codeword(bits: _HeaderBits, n: int):
         bits.put_bits(n - 37, 7)
 
 
-def t1_codestream(img: dict, cb: int = 64) -> bytes:
-    """Raw codestream (.j2k) holding the code-blocks of t1_image(img): one
-    tile, one layer, LRCP, default precincts, code-block style 0."""
+def _pass_new_segment(i: int, style: int) -> bool:
+    """Pass i of a code-block opens a codeword segment (T.800 Table A.19
+    switches: every pass with termination on each pass; with bypass, the
+    first raw pass of each bit-plane and each cleanup from pass 10 on)."""
+    if i == 0 or style & 0x04:
+        return True
+    return bool(style & 0x01) and i >= 10 and i % 3 != 2
+
+
+def _layer_parts(b: dict, style: int, a: int, e: int):
+    """Codeword-segment parts of passes [a, e) of block b: [(passes, bytes)],
+    the bytes of a segment shared out over its passes."""
+    segs = b.get("seg_lens") or [b["len"]]
+    starts = [i for i in range(b["num_pass"]) if _pass_new_segment(i, style)]
+    ends = starts[1:] + [b["num_pass"]]
+    parts, pos = [], 0
+    for s0, s1, ln in zip(starts, ends, segs):
+        lo, hi = max(a, s0), min(e, s1)
+        if lo < hi:
+            c0 = ln * (lo - s0) // (s1 - s0)
+            c1 = ln * (hi - s0) // (s1 - s0)
+            parts.append((hi - lo, pos + c0, c1 - c0))
+        pos += ln
+    return parts
+
+
+def t1_codestream(img: dict, cb: int = 64, layers: int = 1) -> bytes:
+    """Raw codestream (.j2k) holding the code-blocks of t1_image or
+    lossless_image: one tile, LRCP, default precincts, code-block style
+    img["style"] (0 when absent).  With `layers` > 1 every code-block's
+    passes are shared out over the quality layers (the bytes of a codeword
+    segment split between the layers that hold its passes), so packets
+    carry several segment lengths and segments span layers."""
     import struct
     w, h, L, nc, filt = img["w"], img["h"], img["levels"], img["ncomp"], img["filt"]
+    style = img.get("style", 0)
     data = img["data"]
     by_band = {}
     for b in img["blocks"]:
         by_band.setdefault((b["comp"], b["band"]), []).append(b)
     numbps = {b["band"]: b["numbps"] for b in img["blocks"]}
+    bit_depth = img.get("bit_depth", 8)
+    mct = img.get("mct", nc == 3)
     out = bytearray(struct.pack(">H", 0xFF4F))
-    siz = struct.pack(">HIIIIIIIIH", 0, w, h, 0, 0, w, h, 0, 0, nc) + bytes([7, 1, 1]) * nc
+    siz = struct.pack(">HIIIIIIIIH", 0, w, h, 0, 0, w, h, 0, 0, nc) + bytes([bit_depth - 1, 1, 1]) * nc
     out += struct.pack(">HH", 0xFF51, 2 + len(siz)) + siz
     lg = cb.bit_length() - 1
-    cod = struct.pack(">BBHBBBBBB", 0, 0, 1, 1 if nc == 3 else 0, L, lg - 2, lg - 2, 0,
+    cod = struct.pack(">BBHBBBBBB", 0, 0, layers, 1 if mct else 0, L, lg - 2, lg - 2, style,
                       0 if filt == "97" else 1)
     out += struct.pack(">HH", 0xFF52, 2 + len(cod)) + cod
     if filt == "97":        # guard bits 1: QCD value = numbps
         qcd = bytes([(1 << 5) | 2]) + b"".join(struct.pack(">H", numbps[k]) for k in range(3 * L + 1))
+    elif "expn" in img:     # reversible: exponents and guard bits of the encode
+        qcd = bytes([img["guard"] << 5]) + bytes(img["expn"][k] << 3 for k in range(3 * L + 1))
     else:                   # exponent = numbps with guard bits 1
         qcd = bytes([(1 << 5) | 0]) + bytes(numbps[k] << 3 for k in range(3 * L + 1))
     out += struct.pack(">HH", 0xFF5C, 2 + len(qcd)) + qcd
+    # per block: passes of each layer, first layer with passes, Lblock
+    split = {}
+    for b in img["blocks"]:
+        n = b["num_pass"]
+        cuts = [n * k // layers for k in range(layers + 1)]
+        first = next((k for k in range(layers) if cuts[k + 1] > cuts[k]), 1 << 20)
+        split[id(b)] = dict(cuts=cuts, first=first, lblock=3)
+    trees = {}
+    for key, blks in by_band.items():
+        blks.sort(key=lambda b: (b["y"], b["x"]))
+        xs = sorted({b["x"] for b in blks})
+        ys = sorted({b["y"] for b in blks})
+        first = np.array([split[id(b)]["first"] for b in blks]).reshape(len(ys), len(xs))
+        zbp = np.array([b["missing"] if b["num_pass"] else 1 << 20 for b in blks]).reshape(first.shape)
+        trees[key] = (len(xs), _TagTreeWriter(first), _TagTreeWriter(zbp))
     packets = bytearray()
-    for r in range(L + 1):
-        for c in range(nc):
-            bands = [0] if r == 0 else [3 * (r - 1) + o for o in (1, 2, 3)]
-            hb, body, any_incl = _HeaderBits(), bytearray(), False
-            plan = []
-            for bid in bands:
-                blks = sorted(by_band.get((c, bid), []), key=lambda b: (b["y"], b["x"]))
-                if not blks:
-                    plan.append(None)
+    for lay in range(layers):
+        for r in range(L + 1):
+            for c in range(nc):
+                bands = [0] if r == 0 else [3 * (r - 1) + o for o in (1, 2, 3)]
+                hb, body = _HeaderBits(), bytearray()
+                contrib = any(split[id(b)]["cuts"][lay + 1] > split[id(b)]["cuts"][lay]
+                              for bid in bands for b in by_band.get((c, bid), []))
+                if not contrib:
+                    hb.put(0)
+                    packets += hb.finish()
                     continue
-                xs = sorted({b["x"] for b in blks})
-                ys = sorted({b["y"] for b in blks})
-                grid = np.array([[1 if b["num_pass"] == 0 else 0 for b in blks]]).reshape(len(ys), len(xs))
-                zbp = np.array([b["missing"] if b["num_pass"] else 1 << 20 for b in blks]).reshape(grid.shape)
-                plan.append((blks, len(xs), _TagTreeWriter(grid), _TagTreeWriter(zbp)))
-                any_incl |= bool((grid == 0).any())
-            if not any_incl:
-                hb.put(0)
-            else:
                 hb.put(1)
-                for p in plan:
-                    if p is None:
+                for bid in bands:
+                    if (c, bid) not in trees:
                         continue
-                    blks, gw, incl, zbp = p
-                    for k, b in enumerate(blks):
+                    gw, incl, zbp = trees[(c, bid)]
+                    for k, b in enumerate(by_band[(c, bid)]):
                         i, j = divmod(k, gw)
-                        incl.encode(hb, i, j, 1)
-                        if b["num_pass"] == 0:
-                            continue
-                        zbp.encode(hb, i, j, 1 << 30)
-                        _pass_codeword(hb, b["num_pass"])
-                        base = 3 + b["num_pass"].bit_length() - 1
-                        extra = max(0, max(1, b["len"].bit_length()) - base)
+                        st = split[id(b)]
+                        a, e = st["cuts"][lay], st["cuts"][lay + 1]
+                        if st["first"] >= lay:   # not included by an earlier layer
+                            incl.encode(hb, i, j, lay + 1)
+                            if st["first"] != lay:
+                                continue
+                            zbp.encode(hb, i, j, 1 << 30)
+                        else:
+                            hb.put(1 if e > a else 0)
+                            if e == a:
+                                continue
+                        _pass_codeword(hb, e - a)
+                        parts = _layer_parts(b, style, a, e)
+                        need = max(max(1, ln.bit_length()) - (np_.bit_length() - 1)
+                                   for np_, _o, ln in parts)
+                        extra = max(0, need - st["lblock"])
                         hb.put_bits((1 << (extra + 1)) - 2, extra + 1)   # extra ones, then 0
-                        hb.put_bits(b["len"], base + extra)
-                        body += data[b["offset"]:b["offset"] + b["len"]]
-            packets += hb.finish() + body
+                        st["lblock"] += extra
+                        for np_, o, ln in parts:
+                            hb.put_bits(ln, st["lblock"] + np_.bit_length() - 1)
+                            body += data[b["offset"] + o:b["offset"] + o + ln]
+                packets += hb.finish() + body
     sot_len = 12 + 2 + len(packets)
     out += struct.pack(">HHHIBB", 0xFF90, 10, 0, sot_len, 0, 1)
     out += struct.pack(">H", 0xFF93) + packets + struct.pack(">H", 0xFFD9)

@@ -101,6 +101,18 @@ int or_pixel_bytes(int fmt);
  * words (sign bit 31, magnitude below) to out (pitch out_pitch). */
 int or_t1_decode_block(const uint8_t *data, long len, int w, int h, int orient, int numbps,
                        int missing, int num_pass, int32_t *out, int out_pitch);
+/* the same with code-block style `style` (T.800 Table A.19: 0x01 bypass,
+ * 0x02 context reset, 0x04 termination on every pass, 0x08 vertically
+ * causal contexts, 0x10 predictable termination, 0x20 segmentation
+ * symbols).  With bypass or termination on every pass the `len` bytes
+ * hold `nseg` codeword segments of seg_len[k] bytes each, nseg being
+ * or_t1_num_segments(num_pass, style); otherwise one segment.  Returns -1
+ * for a segment table that does not fit. */
+int or_t1_decode_block_ex(const uint8_t *data, long len, int w, int h, int orient, int numbps,
+                          int missing, int num_pass, int style, const int32_t *seg_len, int nseg,
+                          int32_t *out, int out_pitch);
+/* codeword segments that the first num_pass passes of a code-block open */
+int or_t1_num_segments(int num_pass, int style);
 /* dec_cb_stripe's 5/3 conversion (dec_t1_comp.cpp:343-367) */
 void or_t1_words_to_int(const int32_t *words, int w, int h, int pitch, int numbps, int32_t *out,
                         int out_pitch);

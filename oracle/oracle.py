@@ -53,6 +53,8 @@ def lib() -> ctypes.CDLL:
         L.or_ict_float.argtypes = [P, P, P, P, P, P, i, i]
         L.or_pixel_bytes.argtypes = [i]
         L.or_t1_decode_block.argtypes = [P, ctypes.c_long, i, i, i, i, i, i, P, i]
+        L.or_t1_decode_block_ex.argtypes = [P, ctypes.c_long, i, i, i, i, i, i, i, P, i, P, i]
+        L.or_t1_num_segments.argtypes = [i, i]
         L.or_t1_words_to_int.argtypes = [P, i, i, i, i, P, i]
         L.or_t1_tables.argtypes = [P, P, P, P]
         L.or_t1_zc.argtypes = [P, i]
@@ -229,6 +231,25 @@ def t1_decode_block(data: bytes, w: int, h: int, orient: int, numbps: int, missi
                                   _p(out), w)
     assert rc == 0, rc
     return out
+
+
+def t1_decode_block_ex(data: bytes, w: int, h: int, orient: int, numbps: int, missing: int,
+                       num_pass: int, style: int, seg_lens=()) -> np.ndarray:
+    """decode_code_block with code-block style switches: `data` holds the
+    codeword segments of seg_lens (one segment of all bytes unless the
+    style terminates passes)."""
+    buf = np.frombuffer(bytes(data) + b"\0", dtype=np.uint8)
+    segs = np.asarray(list(seg_lens) + [0], dtype=np.int32)
+    out = np.zeros((h, w), dtype=np.int32)
+    rc = lib().or_t1_decode_block_ex(_p(buf), len(data), w, h, orient, numbps, missing, num_pass,
+                                     style, _p(segs), len(seg_lens), _p(out), w)
+    if rc != 0:
+        raise ValueError(f"or_t1_decode_block_ex rejected the block ({rc})")
+    return out
+
+
+def t1_num_segments(num_pass: int, style: int) -> int:
+    return lib().or_t1_num_segments(num_pass, style)
 
 
 def t1_words_to_int(words: np.ndarray, numbps: int) -> np.ndarray:

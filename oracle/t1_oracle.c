@@ -32,6 +32,17 @@
  *     uniform state (MQ_DECODE_START, bmi_mq_decoder.h:131-168).
  * The MQ procedure is written as ITU-T T.800 C.3 (INITDEC, BYTEIN, DECODE,
  * RENORMD); the reference's scaled-register form computes the same symbols.
+ *
+ * or_t1_decode_block_ex adds the code-block style switches of T.800 Table
+ * A.19, which the reference asserts on (bmi_block_decoder.cpp:792-797,
+ * 818-822; its segment loop :729-760 already restarts the MQ decoder per
+ * codeword segment): raw significance / refinement passes from pass 10 on
+ * with bypass (D.6), contexts reset at every pass boundary, a codeword
+ * segment per pass with termall (else, with bypass, segments after pass 9,
+ * per raw pair and per cleanup pass), vertically causal contexts (D.7) and
+ * the segmentation symbol after every cleanup pass (D.5).  These follow
+ * the standard; they are pinned by OpenJPEG decoding the project encoder's
+ * streams to their source images.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -84,7 +95,8 @@ static void bytein(MqDec *m)
     }
 }
 
-static void mq_initdec(MqDec *m, const uint8_t *d, long len)
+/* INITDEC (C.3.5) on a codeword segment; context states are kept */
+static void mq_start(MqDec *m, const uint8_t *d, long len)
 {
     m->d = d;
     m->len = len;
@@ -94,6 +106,11 @@ static void mq_initdec(MqDec *m, const uint8_t *d, long len)
     m->C <<= 7;
     m->CT -= 7;
     m->A = 0x8000;
+}
+
+/* initial context states (bmi_mq_decoder.cpp:60-72) */
+static void mq_reset(MqDec *m)
+{
     for (int i = 0; i < NCX; ++i) m->idx[i] = 0, m->mps[i] = 0;
     m->idx[0] = 4;
     m->idx[CX_RL] = 3;
@@ -190,12 +207,104 @@ static int nb_sig(const uint8_t *f, int s)
     return ((f[-s - 1] | f[-s] | f[-s + 1] | f[-1] | f[1] | f[s - 1] | f[s] | f[s + 1]) & F_SIG) != 0;
 }
 
+/* ---- raw (bypass) segments --------------------------------------------------
+ * T.800 D.6: raw bits most significant first, a byte after 0xFF holds seven
+ * bits (its top bit is stuffed).  Past the end of the segment 0xFF bytes are
+ * read; 0xFF followed by a byte above 0x8F is not consumed (a marker, or
+ * the end), so the reader then returns 1 bits. */
+typedef struct {
+    const uint8_t *d;
+    long len, bp;
+    uint32_t c;
+    int ct;
+} RawDec;
+
+static void raw_start(RawDec *r, const uint8_t *d, long len)
+{
+    r->d = d;
+    r->len = len;
+    r->bp = 0;
+    r->c = 0;
+    r->ct = 0;
+}
+
+static int raw_bit(RawDec *r)
+{
+    if (r->ct == 0) {
+        const uint32_t next = r->bp < r->len ? r->d[r->bp] : 0xFF;
+        if (r->c == 0xFF) {
+            if (next > 0x8F) {
+                r->c = 0xFF;
+                r->ct = 8;
+            } else {
+                r->c = next;
+                r->bp++;
+                r->ct = 7;
+            }
+        } else {
+            r->c = next;
+            r->bp++;
+            r->ct = 8;
+        }
+    }
+    r->ct--;
+    return (int)((r->c >> r->ct) & 1);
+}
+
+/* ---- code-block style (T.800 Table A.19) ------------------------------------ */
+
+enum { CB_BYPASS = 0x01, CB_RESET = 0x02, CB_TERMALL = 0x04, CB_VSC = 0x08, CB_SEGSYM = 0x20 };
+
+/* pass i (0 = first cleanup; type (i + 2) % 3) is raw: bypass, from the
+   fifth coded bit-plane on, significance and refinement passes (D.6) */
+static int pass_raw(int i, int style) { return (style & CB_BYPASS) && i >= 10 && i % 3 != 0; }
+
+/* pass i opens a codeword segment: the first pass; every pass with
+   termination on each pass; with bypass, each raw significance pass and
+   each cleanup pass from pass 10 on (a raw segment holds the significance
+   and refinement passes of one bit-plane) */
+static int pass_new_segment(int i, int style)
+{
+    if (i == 0 || (style & CB_TERMALL)) return 1;
+    return (style & CB_BYPASS) && i >= 10 && i % 3 != 2;
+}
+
+int or_t1_num_segments(int num_pass, int style)
+{
+    int n = 0;
+    for (int i = 0; i < num_pass; ++i) n += pass_new_segment(i, style);
+    return n;
+}
+
+/* 3x3 neighbourhood of the flag grid at f as contexts see it: with
+   vertically causal contexts (D.7) the row below the last row of a stripe
+   counts as insignificant */
+static const uint8_t *view(const uint8_t *f, int s, int causal, uint8_t g[9])
+{
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) g[(dy + 1) * 3 + dx + 1] = (causal && dy > 0) ? 0 : f[dy * s + dx];
+    return g + 4;
+}
+
 /* ---- block decode ---------------------------------------------------------- */
 
-int or_t1_decode_block(const uint8_t *data, long len, int w, int h, int orient, int numbps,
-                       int missing, int num_pass, int32_t *out, int out_pitch)
+int or_t1_decode_block_ex(const uint8_t *data, long len, int w, int h, int orient, int numbps,
+                          int missing, int num_pass, int style, const int32_t *seg_len, int nseg,
+                          int32_t *out, int out_pitch)
 {
-    if (w < 1 || h < 1 || w > 1024 || h > 1024 || orient < 0 || orient > 3 || !out) return -1;
+    if (w < 1 || h < 1 || w > 1024 || h > 1024 || orient < 0 || orient > 3 || !out || (style & ~0x3F))
+        return -1;
+    /* codeword segments: one of `len` bytes unless passes are terminated */
+    const int multi = (style & (CB_BYPASS | CB_TERMALL)) != 0;
+    if (multi) {
+        long sum = 0;
+        if (num_pass < 0 || nseg != or_t1_num_segments(num_pass, style) || (nseg && !seg_len)) return -1;
+        for (int k = 0; k < nseg; ++k) {
+            if (seg_len[k] < 0) return -1;
+            sum += seg_len[k];
+        }
+        if (sum != len) return -1;
+    }
     for (int y = 0; y < h; ++y) memset(out + (long)y * out_pitch, 0, sizeof(int32_t) * w);
     /* :610-624 */
     const int max_original_passes = (numbps - missing) * 3 - 2;
@@ -210,6 +319,7 @@ int or_t1_decode_block(const uint8_t *data, long len, int w, int h, int orient, 
     if (plane == 0) plane = 1;
 
     const int s = w + 2;
+    const int vsc = (style & CB_VSC) != 0;
     uint8_t *fl = (uint8_t *)calloc((size_t)s * (h + 2), 1);
     uint32_t *val = (uint32_t *)calloc((size_t)w * h, sizeof(uint32_t));
     if (!fl || !val) {
@@ -218,7 +328,10 @@ int or_t1_decode_block(const uint8_t *data, long len, int w, int h, int orient, 
         return -1;
     }
     MqDec m;
-    mq_initdec(&m, data, len);
+    RawDec r;
+    mq_reset(&m);
+    long seg_pos = 0;
+    int seg = 0, raw = 0;
     int type = 2;
     for (int pi = 0; pi < num_passes; ++pi, ++type) {
         if (type == 3) {
@@ -229,6 +342,17 @@ int or_t1_decode_block(const uint8_t *data, long len, int w, int h, int orient, 
            would shift by a negative count, undefined in the reference:
            such passes are not decoded */
         if (plane < 0) break;
+        if (pass_new_segment(pi, style)) {
+            const long n = multi ? seg_len[seg] : len;
+            raw = pass_raw(pi, style);
+            if (raw)
+                raw_start(&r, data + seg_pos, n);
+            else
+                mq_start(&m, data + seg_pos, n);
+            seg_pos += n;
+            ++seg;
+        }
+        if ((style & CB_RESET) && pi > 0) mq_reset(&m);
         const uint32_t one_half = (plane >= 0 && plane < 31)
                                       ? ((uint32_t)1 << plane) + (((uint32_t)1 << plane) >> 1)
                                       : 0;
@@ -239,59 +363,83 @@ int or_t1_decode_block(const uint8_t *data, long len, int w, int h, int orient, 
                 int y = y0;
                 if (type == 2 && rows == 4) {
                     int run = 1;
-                    for (int r = 0; r < 4 && run; ++r) {
-                        const uint8_t *f = &fl[(y0 + r + 1) * s + x + 1];
-                        if ((*f & (F_SIG | F_VISIT)) || nb_sig(f, s)) run = 0;
+                    for (int k = 0; k < 4 && run; ++k) {
+                        uint8_t g[9];
+                        const uint8_t *f = &fl[(y0 + k + 1) * s + x + 1];
+                        if ((*f & (F_SIG | F_VISIT)) || nb_sig(view(f, s, vsc && k == 3, g), 3)) run = 0;
                     }
                     if (run) {
                         if (!mq_decode(&m, CX_RL)) continue;
-                        int r = mq_decode(&m, CX_UNI) << 1;
-                        r |= mq_decode(&m, CX_UNI);
-                        uint8_t *f = &fl[(y0 + r + 1) * s + x + 1];
+                        int k = mq_decode(&m, CX_UNI) << 1;
+                        k |= mq_decode(&m, CX_UNI);
+                        uint8_t *f = &fl[(y0 + k + 1) * s + x + 1];
+                        uint8_t g[9];
                         int xr;
-                        const int cx = sc_ctx(f, s, &xr);
+                        const int cx = sc_ctx(view(f, s, vsc && k == 3, g), 3, &xr);
                         const int neg = mq_decode(&m, cx) ^ xr;
                         *f |= F_SIG | (neg ? F_NEG : 0);
-                        val[(long)(y0 + r) * w + x] = ((uint32_t)neg << 31) + one_half;
-                        y = y0 + r + 1;
+                        val[(long)(y0 + k) * w + x] = ((uint32_t)neg << 31) + one_half;
+                        y = y0 + k + 1;
                     }
                 }
                 for (; y < y0 + rows; ++y) {
                     uint8_t *f = &fl[(y + 1) * s + x + 1];
+                    uint8_t g[9];
+                    const uint8_t *n = view(f, s, vsc && (y & 3) == 3, g);
                     uint32_t *v = &val[(long)y * w + x];
                     if (type == 1) {
                         if (!(*f & F_MU)) continue;
                         const uint32_t mag = (*v & 0x7FFFFFFFu) >> plane;
-                        const int cx = mag < 4 ? CX_MR + nb_sig(f, s) : CX_MR + 2;
-                        const int bit = mq_decode(&m, cx);
+                        const int bit = raw ? raw_bit(&r) : mq_decode(&m, mag < 4 ? CX_MR + nb_sig(n, 3) : CX_MR + 2);
                         *v ^= (uint32_t)(1 - bit) << plane;
                         *v |= half;
                         continue;
                     }
                     if (type == 0) {
-                        if ((*f & F_SIG) || !nb_sig(f, s)) continue;
+                        if ((*f & F_SIG) || !nb_sig(n, 3)) continue;
                         *f |= F_VISIT;
                     } else if (*f & (F_SIG | F_VISIT)) {
                         continue;
                     }
-                    if (!mq_decode(&m, zc_ctx(f, s, orient))) continue;
-                    int xr;
-                    const int cx = sc_ctx(f, s, &xr);
-                    const int neg = mq_decode(&m, cx) ^ xr;
+                    int neg;
+                    if (raw) {
+                        /* raw significance bit, then the sign bit itself */
+                        if (!raw_bit(&r)) continue;
+                        neg = raw_bit(&r);
+                    } else {
+                        if (!mq_decode(&m, zc_ctx(n, 3, orient))) continue;
+                        int xr;
+                        const int cx = sc_ctx(n, 3, &xr);
+                        neg = mq_decode(&m, cx) ^ xr;
+                    }
                     *f |= F_SIG | (neg ? F_NEG : 0);
                     *v = ((uint32_t)neg << 31) + one_half;
                 }
             }
         }
-        if (type == 2) /* end of cleanup: significant samples become refinable */
+        if (type == 2) {
+            /* segmentation symbol (D.5): four decisions in the uniform
+               context after every cleanup pass (1010 when intact) */
+            if (style & CB_SEGSYM)
+                for (int k = 0; k < 4; ++k) (void)mq_decode(&m, CX_UNI);
+            /* end of cleanup: significant samples become refinable */
             for (long i = 0; i < (long)s * (h + 2); ++i)
                 fl[i] = (uint8_t)((fl[i] & ~F_VISIT) | ((fl[i] & F_SIG) ? F_MU : 0));
+        }
     }
     for (int y = 0; y < h; ++y)
         memcpy(out + (long)y * out_pitch, val + (long)y * w, sizeof(int32_t) * w);
     free(fl);
     free(val);
     return 0;
+}
+
+/* code-block style 0: the reference's decode_code_block */
+int or_t1_decode_block(const uint8_t *data, long len, int w, int h, int orient, int numbps,
+                       int missing, int num_pass, int32_t *out, int out_pitch)
+{
+    return or_t1_decode_block_ex(data, len, w, h, orient, numbps, missing, num_pass, 0, NULL, 0, out,
+                                 out_pitch);
 }
 
 /* dec_t1_comp.cpp:343-367: T1 words of a 5/3 band to two's complement */

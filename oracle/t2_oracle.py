@@ -14,6 +14,10 @@ restates, function by function:
   - progression orders ...... dec_packet.cpp:384-730
   - numbps of a band ........ decoder/codec/src/dec_jp2.cpp:177-178
   - multi-layer segments .... decoder/codec/src/bmi_block_decoder.cpp:742-760
+  - code-block styles ....... ITU-T T.800 Table A.19, B.10.7.2 (codeword
+                              segments with bypass / termination on every
+                              pass; the reference asserts on them,
+                              bmi_block_decoder.cpp:792-797, 818-822)
   - 9/7 step (mfAbsStepNorm)  decoder/cudainterface/src/cudainterface.cpp:712-720
 
 Where the reference is narrower than ITU-T T.800 this restatement follows
@@ -147,7 +151,8 @@ def parse(buf: bytes, guard_t800: bool = False) -> dict:
     Returns dict(siz=..., tiles=[dict(tx0, ty0, tx1, ty1, levels, filt, mct,
     ncomp, steps, blocks=[...]) ...]).  Each block: comp, band, x, y (Mallat
     position in the tile-component plane), w, h, orient, numbps, missing,
-    num_pass, data (its layer chunks concatenated)."""
+    num_pass, style (code-block style bits), data (its codeword segments
+    concatenated, each gathered over the layers) and seg_lens."""
     cs = codestream_of(buf)
     if _u(cs, 0, 2) != MK_SOC:
         raise ValueError("no SOC")
@@ -368,7 +373,7 @@ def _decode_tile(siz, t, ntx, main_cod, main_coc, main_qcd, main_qcc, T, guard_t
     comps = []
     for c in range(ncomp):
         cod, qc = _tile_params(siz, T, main_cod, main_coc, main_qcd, main_qcc, c)
-        if cod["cbstyle"] & ~0x10:
+        if cod["cbstyle"] & ~0x3F:
             raise Unsupported(f"code-block style {cod['cbstyle']:#x}")
         if cod["xcb"] > 6 or cod["ycb"] > 6:
             raise Unsupported("code-blocks over 64 samples a side")
@@ -385,7 +390,8 @@ def _decode_tile(siz, t, ntx, main_cod, main_coc, main_qcd, main_qcc, T, guard_t
             blocks.append(dict(comp=cp["c"], band=b["band"], x=b["x"], y=b["y"], w=b["w"],
                                h=b["h"], orient=b["orient"], numbps=b["numbps"],
                                missing=b["missing"], num_pass=b["num_pass"],
-                               data=b"".join(b["chunks"])))
+                               style=b["style"], data=b"".join(b["segs"]),
+                               seg_lens=[len(x) for x in b["segs"]]))
     tile["blocks"] = blocks
     tile["steps"] = [cp["steps"] for cp in comps]
     return tile
@@ -440,7 +446,7 @@ def _comp_geometry(tx0, ty0, tx1, ty1, cod, qc, depth, c, guard_t800=False):
                     y1 = min((gy0 + j + 1) << ycb, tby1)
                     b = dict(band=bid, orient=o, x=x0 - tbx0 + llw * xob, y=y0 - tby0 + llh * yob,
                              w=x1 - x0, h=y1 - y0, numbps=numbps, missing=0, num_pass=0,
-                             lblock=3, incl=False, chunks=[])
+                             lblock=3, incl=False, segs=[], style=cod["cbstyle"])
                     grid[(i, j)] = b
                     blocks.append(b)
             # precincts of this band (dec_init.cpp:963-1008)
@@ -475,6 +481,32 @@ def _abs_step_norm(exp, mant, depth):
 
 # -------------------------------------------------------------- packets --
 
+def pass_new_segment(i: int, style: int) -> bool:
+    """Coding pass i of a code-block (0 = the first cleanup pass) opens a
+    codeword segment (T.800 Table A.19, D.6): the first pass; every pass
+    when each pass is terminated; with bypass, from pass 10 on each raw
+    significance pass and each cleanup pass (the raw segment of a
+    bit-plane holds its significance and refinement passes)."""
+    if i == 0 or style & 0x04:
+        return True
+    return bool(style & 0x01) and i >= 10 and i % 3 != 2
+
+
+def _segment_parts(p0: int, n: int, style: int):
+    """[(segment index, passes)] of the passes [p0, p0 + n)."""
+    k = sum(pass_new_segment(i, style) for i in range(p0 + 1)) - 1
+    parts = []
+    for i in range(p0, p0 + n):
+        if i > p0 and pass_new_segment(i, style):
+            k += 1
+        if parts and parts[-1][0] == k:
+            parts[-1][1] += 1
+        else:
+            parts.append([k, 1])
+    return [tuple(x) for x in parts]
+
+
+
 def _read_packet(bits: _Bits, data: bytes, res: dict, pi: int, layer: int, sop: bool, eph: bool,
                  log: list):
     """dec_read_packet + dec_prec + dec_restore_coding_pass
@@ -501,12 +533,16 @@ def _read_packet(bits: _Bits, data: bytes, res: dict, pi: int, layer: int, sop: 
                         raise ValueError("missing bit-planes > 74")
                 cb["incl"] = True
                 n = _num_passes(bits)
+                p0 = cb["num_pass"]
                 cb["num_pass"] += n
                 if cb["num_pass"] > 128:
                     raise ValueError("more than 128 coding passes")
                 while bits.bit():
                     cb["lblock"] += 1
-                cb["_new"] = bits.int(cb["lblock"] + int(math.floor(math.log2(n))))
+                # one length per codeword segment the new passes touch,
+                # Lblock + floor(log2(passes of that segment)) bits each
+                cb["_new"] = [(k, bits.int(cb["lblock"] + int(math.floor(math.log2(m)))))
+                              for k, m in _segment_parts(p0, n, cb["style"])]
     bits.align()
     hdr_end = bits.cur
     if eph and bits.cur + 1 < len(data) and _u(data, bits.cur, 2) == MK_EPH:
@@ -518,11 +554,12 @@ def _read_packet(bits: _Bits, data: bytes, res: dict, pi: int, layer: int, sop: 
         if pr is None:
             continue
         for cb in pr["cbs"]:
-            n = cb.pop("_new", 0)
-            if n:
+            for k, n in cb.pop("_new", ()):
                 if bits.cur + n > len(data):
                     raise ValueError("code-block data past the end of the tile")
-                cb["chunks"].append(data[bits.cur:bits.cur + n])
+                while len(cb["segs"]) <= k:
+                    cb["segs"].append(b"")
+                cb["segs"][k] += data[bits.cur:bits.cur + n]
                 bits.cur += n
     log.append((hdr_end, bits.cur))
 
@@ -630,8 +667,8 @@ def tile_planes(tile, orc):
     w, h = tile["tx1"] - tile["tx0"], tile["ty1"] - tile["ty0"]
     planes = np.zeros((tile["ncomp"], h, w), dtype=np.int32)
     for b in tile["blocks"]:
-        words = orc.t1_decode_block(b["data"], b["w"], b["h"], b["orient"], b["numbps"],
-                                    b["missing"], b["num_pass"])
+        words = orc.t1_decode_block_ex(b["data"], b["w"], b["h"], b["orient"], b["numbps"],
+                                       b["missing"], b["num_pass"], b["style"], b["seg_lens"])
         if tile["filt"] == "53":
             words = orc.t1_words_to_int(words, b["numbps"])
         planes[b["comp"], b["y"]:b["y"] + b["h"], b["x"]:b["x"] + b["w"]] = words

@@ -283,7 +283,9 @@ def _compare(pkg, data: bytes, flags: int = 0) -> int:
         ptrs = [base * (c + 1) for c in range(T["ncomp"])]
         pitch = w + 3
         blocks, segs, desc = cs.tile_blocks(t, ptrs, pitch, seg_base=0)
-        assert sb == sum(len(b["data"]) for b in T["blocks"])
+        segmented = [b for b in T["blocks"] if b["style"] & 5]
+        tables = -(-4 * sum(len(b["seg_lens"]) for b in segmented) // 16) * 16
+        assert sb == tables + sum(len(b["data"]) for b in T["blocks"])
         assert segs[sb:sb + 16].tobytes() == b"\xff" * 16
         prod = {}
         for i in range(nb):
@@ -291,11 +293,22 @@ def _compare(pkg, data: bytes, flags: int = 0) -> int:
             c = (b.out // base) - 1
             y, x = divmod((b.out - ptrs[c]) // 4, pitch)
             assert b.out_pitch == pitch
+            lens = ()
+            if b.style & 5:
+                assert b.seg_offset % 4 == 0 and b.seg_offset + 4 * b.nseg <= tables
+                lens = tuple(np.frombuffer(segs[b.seg_offset:b.seg_offset + 4 * b.nseg].tobytes(),
+                                           np.int32).tolist())
+            else:
+                assert b.nseg == 1
+            assert b.data_offset >= tables
             prod[(c, x, y)] = (b.width, b.height, b.orient, b.numbps, b.missing, b.num_pass,
-                               b.kind, segs[b.data_offset:b.data_offset + b.len].tobytes())
+                               b.kind, segs[b.data_offset:b.data_offset + b.len].tobytes(),
+                               b.style, lens)
         kind = 1 if T["filt"] == "53" else 0
         orc_tab = {(b["comp"], b["x"], b["y"]): (b["w"], b["h"], b["orient"], b["numbps"],
-                                                  b["missing"], b["num_pass"], kind, b["data"])
+                                                  b["missing"], b["num_pass"], kind, b["data"],
+                                                  b["style"],
+                                                  tuple(b["seg_lens"]) if b["style"] & 5 else ())
                    for b in T["blocks"]}
         assert prod == orc_tab
         L = T["levels"]
@@ -382,11 +395,13 @@ def test_product_rejects_malformed_and_unsupported(pkg):
         payload = b"\x00" * (7 if marker == t2.MK_POC else 3)
         bad = jt.rewrite(data, extra_main=struct.pack(">HH", marker, len(payload) + 2) + payload)
         _expect_error(pkg, bad, pkg.BMI_E_UNSUPPORTED)
-    # code-block styles: bypass (1), reset (2), termall (4), causal (8), segsym (32)
-    for style in (1, 2, 4, 8, 32):
+    # code-block styles beyond T.800 Part 1 (0x40 high-throughput, 0x80)
+    for style in (0x40, 0x80, 0x41):
         _expect_error(pkg, _set_cod_byte(data, 8, style), pkg.BMI_E_UNSUPPORTED)
-    # predictable termination changes nothing for the decoder
-    assert pkg.Codestream(_set_cod_byte(data, 8, 16)).info.total_blocks > 0
+    # reset, causal, predictable termination and segmentation symbols leave
+    # the packet headers as they are
+    for style in (2, 8, 16, 32):
+        assert pkg.Codestream(_set_cod_byte(data, 8, style)).info.total_blocks > 0
     # 128x32 code-blocks: legal (area 4096), wider than the GPU decoder's 64
     _expect_error(pkg, _set_cod_byte(_set_cod_byte(data, 7, 3), 6, 5), pkg.BMI_E_UNSUPPORTED)
     # 128x64: area over 4096, malformed
@@ -460,3 +475,86 @@ def test_own_writer_round_trip(pkg, workloads, w, h, levels, nc, filt, cb):
     assert got == exp
     assert np.array_equal(np.asarray(T["steps"], np.float32), img["steps"])
     _compare(pkg, data)
+
+
+# ------------------------------------------------------ code-block styles --
+# Streams with every code-block style switch (T.800 Table A.19), written by
+# the project's own encoder (workloads.lossless_image + t1_codestream: RCT,
+# forward 5/3, tier-1 with the style, packet headers with one length per
+# codeword segment, several layers).  OpenJPEG (through Pillow) decoding
+# them to the source image pins the encoder and the segment signalling
+# independently; the oracle and the product parser must then read them
+# back exactly.
+
+STYLE_CASES = [0x01, 0x02, 0x04, 0x08, 0x10, 0x20, 0x05, 0x09, 0x21, 0x3F, 0x2F, 0x1B]
+
+
+def _style_image(rng, h, w, nc):
+    img = rng.integers(0, 256, (h, w, nc), dtype=np.uint8)
+    return img if nc == 3 else img[..., 0]
+
+
+@pytest.mark.parametrize("style", STYLE_CASES, ids=lambda s: f"style{s:#04x}")
+@pytest.mark.parametrize("layers", [1, 3])
+def test_style_streams_decode_exactly(pkg, orc, workloads, style, layers):
+    Image = pytest.importorskip("PIL.Image")
+    rng = np.random.default_rng(style * 7 + layers)
+    for (h, w, nc, L, cb) in [(70, 90, 3, 3, 32), (33, 47, 1, 2, 16)]:
+        src = _style_image(rng, h, w, nc)
+        img = workloads.lossless_image(src, L, cb=cb, style=style)
+        data = workloads.t1_codestream(img, cb=cb, layers=layers)
+        # independent decoder: OpenJPEG
+        opj = np.asarray(Image.open(io.BytesIO(data)))
+        assert np.array_equal(opj.reshape(src.shape), src), (style, layers, h, w)
+        # oracle tier 2 + tier 1 with the style, T.800 5/3 synthesis, RCT
+        P = t2.parse(data)
+        assert all(b["style"] == style for b in P["tiles"][0]["blocks"])
+        planes = t2.decode_image(data, orc, out_format=7, r53_mode=orc.R53_TEXTBOOK)
+        got = np.stack(_rct_dc(planes, P["tiles"][0]["mct"], 8), axis=-1)
+        assert np.array_equal(got, src.reshape(got.shape)), (style, layers, h, w)
+        # product parse: same blocks, segments and segment-length tables
+        _compare(pkg, data)
+        # the blocks the writer encoded come back segment for segment
+        T = P["tiles"][0]
+        exp = {(b["comp"], b["x"], b["y"]): (b["num_pass"], b["missing"],
+                                             img["data"][b["offset"]:b["offset"] + b["len"]],
+                                             tuple(b["seg_lens"]))
+               for b in img["blocks"]}
+        got_b = {(b["comp"], b["x"], b["y"]): (b["num_pass"], b["missing"], b["data"],
+                                               tuple(b["seg_lens"]))
+                 for b in T["blocks"]}
+        assert got_b == exp
+
+
+def test_style_segments_span_layers(pkg, workloads):
+    """With bypass and three layers some codeword segments are split over
+    two packets and some packets carry several segment lengths of one
+    block: the parsers gather each segment over the layers."""
+    rng = np.random.default_rng(11)
+    src = _style_image(rng, 64, 64, 1)
+    img = workloads.lossless_image(src, 2, cb=64, style=0x01)
+    data = workloads.t1_codestream(img, cb=64, layers=3)
+    spans = multi = 0
+    for b in img["blocks"]:
+        n = b["num_pass"]
+        cuts = [n * k // 3 for k in range(4)]
+        starts = [i for i in range(n) if t2.pass_new_segment(i, 0x01)] + [n]
+        for s0, s1 in zip(starts, starts[1:]):
+            spans += sum(1 for c in cuts[1:3] if s0 < c < s1)
+        for a, e in zip(cuts, cuts[1:]):
+            multi += len({k for k in range(len(starts) - 1) if starts[k] < e and starts[k + 1] > a}) > 1
+    assert spans > 0 and multi > 0
+    _compare(pkg, data)
+
+
+def test_segment_table_helpers_agree(pkg, orc):
+    """Segment counts of the oracle, the product and the writer's rule for
+    every style and pass count; bypass without termall: 10 passes, then
+    raw pairs and single cleanup passes."""
+    for style in range(64):
+        for n in range(0, 40):
+            want = sum(t2.pass_new_segment(i, style) for i in range(n))
+            assert orc.t1_num_segments(n, style) == want
+            assert pkg.lib().BMI_t1_num_segments(n, style) == want
+    assert [t2.pass_new_segment(i, 1) for i in range(16)] == \
+        [True] + [False] * 9 + [True, False, True, True, False, True]

@@ -12,7 +12,9 @@ one batched IDWT / colour plan over all tiles.
     planes bit for bit;
   - streams rewritten with SOP / EPH markers, tile-parts and truncated
     tiles decode as the oracle does; a decoder object re-runs to the same
-    output."""
+    output;
+  - streams of every code-block style switch (bypass, reset, termall,
+    causal, segmentation symbols) decode exactly."""
 import pathlib
 
 import numpy as np
@@ -156,3 +158,34 @@ def test_gpu_codestream_reduced_resolution(pkg, orc, name, top):
     assert got.shape == exp.shape
     assert np.array_equal(got, exp), (name, top)
     dec.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("style", [0x01, 0x04, 0x05, 0x08, 0x22, 0x3F], ids=lambda s: f"style{s:#04x}")
+def test_gpu_style_codestreams(pkg, orc, workloads, style):
+    """Streams of every code-block style switch from the project's encoder
+    (OpenJPEG decodes them to the source: tests/test_codestream.py): the
+    GPU decode is exact against the source with the T.800 5/3 synthesis and
+    equals the oracle's packed decode in the reference mode; three layers,
+    so codeword segments span packets."""
+    rng = np.random.default_rng(style)
+    for (h, w, nc, L, cb) in [(70, 90, 3, 3, 32), (130, 67, 1, 4, 64)]:
+        src = rng.integers(0, 256, (h, w, nc), dtype=np.uint8)
+        if nc == 1:
+            src = src[..., 0]
+        img = workloads.lossless_image(src, L, cb=cb, style=style)
+        data = workloads.t1_codestream(img, cb=cb, layers=3)
+        print(f"case style {style:#x} {h}x{w}x{nc}", flush=True)
+        planes = _gpu_planes(pkg, data, r53_mode=pkg.R53_TEXTBOOK)
+        p = [x.astype(np.int64) for x in planes]
+        if nc == 3:
+            y0, y1, y2 = p
+            g = y0 - ((y1 + y2) >> 2)
+            p = [y2 + g, g, y1 + g]
+        got = np.stack([x + 128 for x in p], axis=-1)
+        assert np.array_equal(got, src.reshape(got.shape)), (style, h, w)
+        cs = pkg.Codestream(data)
+        fmt, co = pkg.default_format(cs.info)
+        packed = _gpu_packed(pkg, data)
+        exp = t2.decode_image(data, orc, out_format=fmt, check_out=co)
+        assert np.array_equal(packed, exp), (style, h, w)

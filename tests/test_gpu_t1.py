@@ -103,6 +103,90 @@ def test_random_code_blocks_match_oracle(pkg, workloads, orc):
     assert not bad, f"{len(bad)} blocks differ, first {bad[:5]}"
 
 
+def test_style_code_blocks_match_oracle(pkg, workloads, orc):
+    """Every code-block style combination (T.800 Table A.19: bypass, reset,
+    termall, causal, predictable termination, segmentation symbols) on
+    blocks of several shapes, whole and truncated: GPU words == oracle
+    words.  Segmented styles carry their segment-length tables inside the
+    data buffer (seg_offset)."""
+    import torch
+    rng = np.random.default_rng(2024)
+    sizes = [(64, 64), (13, 29), (4, 4), (1, 64), (64, 1), (33, 64), (7, 5)]
+    cases = []
+    for style in range(64):
+        for j in range(3):
+            h, w = sizes[(style + 3 * j) % len(sizes)]
+            q = np.round(rng.laplace(0, [4.0, 40.0, 400.0][j], (h, w))).astype(np.int32)
+            K = int(np.abs(q).max()).bit_length()
+            mb = max(K, 1) + int(rng.integers(0, 2))
+            limit = int(rng.integers(1, 3 * K - 2)) if (K > 4 and j == 2) else 0
+            orient = int(rng.integers(0, 4))
+            data, npass, miss, segs = workloads.t1_encode_block(q, mb, orient, limit, style=style,
+                                                                segments=True)
+            kind = (style + j) % 2
+            numbps = ((mb << 11) | int(rng.integers(0, 2048))) if kind == 0 else mb
+            cases.append(dict(h=h, w=w, orient=orient, kind=kind, numbps=numbps, missing=miss,
+                              num_pass=npass, data=data, style=style, segs=segs))
+    tables = [np.asarray(c["segs"], np.int32) for c in cases]
+    tab_bytes = -(-sum(4 * len(t) for t in tables) // 16) * 16
+    pitch = 64 * 16
+    rows = (len(cases) + 15) // 16
+    out = torch.full((rows * 64, pitch), -7, dtype=torch.int32, device="cuda")
+    blocks, chunks, off, toff = [], [], tab_bytes, 0
+    for i, c in enumerate(cases):
+        bx, by = 64 * (i % 16), 64 * (i // 16)
+        b = pkg.BmiT1Block()
+        b.data_offset, b.len = off, len(c["data"])
+        b.width, b.height, b.orient = c["w"], c["h"], c["orient"]
+        b.numbps, b.missing, b.num_pass, b.kind = c["numbps"], c["missing"], c["num_pass"], c["kind"]
+        b.out = out.data_ptr() + 4 * (by * pitch + bx)
+        b.out_pitch = pitch
+        b.style = c["style"]
+        if c["style"] & 5:
+            b.nseg, b.seg_offset = len(c["segs"]), toff
+            toff += 4 * len(c["segs"])
+        else:
+            b.nseg = 1
+        blocks.append(b)
+        chunks.append(c["data"])
+        off += len(c["data"])
+    table = b"".join(tables[i].tobytes() for i, c in enumerate(cases) if c["style"] & 5)
+    pkg.t1_check_blocks(blocks)
+    dblocks = upload_blocks(pkg, blocks)
+    ddata = upload_data(table + b"\0" * (tab_bytes - len(table)) + b"".join(chunks))
+    torch.cuda.synchronize()
+    print(f"decoding {len(blocks)} style code-blocks, {off} bytes", flush=True)
+    pkg.t1_decode_blocks(dblocks.data_ptr(), len(blocks), ddata.data_ptr(), 0)
+    got_all = out.cpu().numpy()
+    bad = []
+    for i, c in enumerate(cases):
+        bx, by = 64 * (i % 16), 64 * (i // 16)
+        got = got_all[by:by + c["h"], bx:bx + c["w"]]
+        exp = orc.t1_decode_block_ex(c["data"], c["w"], c["h"], c["orient"], c["numbps"],
+                                     c["missing"], c["num_pass"], c["style"], c["segs"])
+        if c["kind"] == 1:
+            exp = orc.t1_words_to_int(exp, c["numbps"])
+        if not np.array_equal(got, exp):
+            bad.append((i, c["style"], c["h"], c["w"], c["orient"], c["kind"], c["num_pass"]))
+    assert not bad, f"{len(bad)} blocks differ, first {bad[:5]}"
+
+
+def test_check_blocks_rejects_styles(pkg):
+    b = pkg.BmiT1Block()
+    b.width, b.height, b.out, b.out_pitch, b.kind, b.num_pass = 8, 8, 1, 8, 0, 12
+    b.style = 0x40                       # not a Part 1 switch
+    with pytest.raises(pkg.BmiError):
+        pkg.t1_check_blocks([b])
+    b.style, b.nseg = 0x04, 11           # termall: 12 passes open 12 segments
+    with pytest.raises(pkg.BmiError):
+        pkg.t1_check_blocks([b])
+    b.nseg, b.seg_offset = 12, 2         # table not 4-byte aligned
+    with pytest.raises(pkg.BmiError):
+        pkg.t1_check_blocks([b])
+    b.seg_offset = 8
+    pkg.t1_check_blocks([b])
+
+
 def test_check_blocks_rejects(pkg):
     b = pkg.BmiT1Block()
     b.width, b.height, b.out, b.out_pitch, b.kind = 65, 4, 1, 128, 0
