@@ -123,10 +123,12 @@ struct Emitter {
     __attribute__((address_space(1))) uint8_t *out;
     int pitch, a0, b0, ox, oy, bd, od, ls, rs, mx, check_out, fmt, nc;
     float fmx, foff;
-    bool pair_fast;  // both pixels of a lane in one aligned store sequence
+    bool pair_fast;      // both pixels of a lane in one aligned store sequence
+    bool uniform_pairs;  // every lane stores both of its pixels or neither
 
     __device__ inline void init(const LevelJob &__restrict__ J)
     {
+        uniform_pairs = false;
         a0 = J.a0;
         b0 = J.b0;
         nc = J.nc;
@@ -156,9 +158,77 @@ struct Emitter {
         }
     }
 
-    __device__ inline int lift(int v) const { return (v << ls) >> rs; }
+    template <bool SHIFT = true>
+    __device__ inline int lift(int v) const
+    {
+        if constexpr (SHIFT) return (v << ls) >> rs;
+        else return v;
+    }
 
-    // colour of one pixel (before packing)
+    // the lanes' output masks of an item: when no lane stores just one of
+    // its two pixels (all but the odd right edge of a tile), put2 takes the
+    // paired path in every lane and masks only the stores
+    __device__ inline void set_lanes(bool e_ok, bool o_ok)
+    {
+        uniform_pairs = __builtin_amdgcn_ballot_w64(e_ok != o_ok) == 0;
+    }
+
+    // put2_full applies (rows inside the level, paired lanes, no element
+    // shift, no grey check-out): the common case of an item, decided once
+    __device__ inline bool full_ok() const
+    {
+        if constexpr (OUTK == K_PLANE) return pair_fast && uniform_pairs;
+        else if constexpr (PL == PL_BGR8) return pair_fast && uniform_pairs && ls == 0 && rs == 0 && check_out < 0;
+        else return false;
+    }
+
+    // put2 for items where full_ok() holds: no branch but the store mask
+    __device__ inline void put2_full(int row, int ce, const T *ve, const T *vo, bool e_ok) const
+    {
+        if constexpr (OUTK == K_PLANE) {
+            const uint32_t off = ((uint32_t)(row - b0) * (uint32_t)pitch + (uint32_t)(ce - a0)) * 4u;
+            if (e_ok) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    typedef T v2 __attribute__((ext_vector_type(2)));
+                    v2 pairv;
+                    pairv.x = ve[c];
+                    pairv.y = vo[c];
+                    *(__attribute__((address_space(1))) v2 *)at(dst[c], off) = pairv;
+                }
+            }
+        } else if constexpr (PL == PL_BGR8) {
+            int R0, G0, B0, A0, R1, G1, B1, A1;
+            colour<false, false>(ve, R0, G0, B0, A0);
+            colour<false, false>(vo, R1, G1, B1, A1);
+            store_pair_bgr8(row, ce, R0, G0, B0, R1, G1, B1, e_ok);
+        }
+    }
+
+    // B0 G0 R0 B1 G1 R1 at an even pixel (a 2-aligned byte offset): one
+    // aligned dword and one aligned short
+    __device__ inline void store_pair_bgr8(int row, int ce, int R0, int G0, int B0, int R1, int G1, int B1,
+                                           bool e_ok) const
+    {
+        const uint32_t off = ((uint32_t)(oy + row) * (uint32_t)pitch + (uint32_t)(ox + ce)) * 3u;
+        const bool mid = (off & 2) != 0;
+        const uint32_t lo4 = (uint32_t)(B0 & 0xff) | ((uint32_t)(G0 & 0xff) << 8) |
+                             ((uint32_t)(R0 & 0xff) << 16) | ((uint32_t)(B1 & 0xff) << 24);
+        const uint32_t hi4 = (uint32_t)(R0 & 0xff) | ((uint32_t)(B1 & 0xff) << 8) |
+                             ((uint32_t)(G1 & 0xff) << 16) | ((uint32_t)(R1 & 0xff) << 24);
+        const uint32_t sh_lo = (uint32_t)(B0 & 0xff) | ((uint32_t)(G0 & 0xff) << 8);
+        const uint32_t sh_hi = (uint32_t)(G1 & 0xff) | ((uint32_t)(R1 & 0xff) << 8);
+        const uint32_t dw_off = mid ? off + 2 : off;
+        const uint32_t sh_off = mid ? off : off + 4;
+        if (e_ok) {
+            *(__attribute__((address_space(1))) uint32_t *)at(out, dw_off) = mid ? hi4 : lo4;
+            *(__attribute__((address_space(1))) uint16_t *)at(out, sh_off) = (uint16_t)(mid ? sh_lo : sh_hi);
+        }
+    }
+
+    // colour of one pixel (before packing); SHIFT false when the output
+    // element depth equals the bit depth (ls = rs = 0)
+    template <bool SHIFT = true, bool CHECK = true>
     __device__ inline void colour(const T *v, int &R, int &G, int &B, int &A) const
     {
         A = 0;
@@ -170,19 +240,22 @@ struct Emitter {
                     const float g = __fadd_rn(
                         __fadd_rn(__fadd_rn(v[0], __fmul_rn(kCbG, v[1])), __fmul_rn(kCrG, v[2])), foff);
                     const float b = __fadd_rn(__fadd_rn(v[0], __fmul_rn(kCbB, v[1])), foff);
-                    R = lift((int)__builtin_amdgcn_fmed3f(r, 0.0f, fmx));
-                    G = lift((int)__builtin_amdgcn_fmed3f(g, 0.0f, fmx));
-                    B = lift((int)__builtin_amdgcn_fmed3f(b, 0.0f, fmx));
+                    R = lift<SHIFT>((int)__builtin_amdgcn_fmed3f(r, 0.0f, fmx));
+                    G = lift<SHIFT>((int)__builtin_amdgcn_fmed3f(g, 0.0f, fmx));
+                    B = lift<SHIFT>((int)__builtin_amdgcn_fmed3f(b, 0.0f, fmx));
                 } else {
                     // RRct_53, bmi_cpu_dwt_decoder.cpp:3936-4130
                     const int g = v[0] - ((v[1] + v[2]) >> 2) + (1 << (bd - 1));
-                    R = lift(min(max(v[2] + g, 0), mx));
-                    G = lift(min(max(g, 0), mx));
-                    B = lift(min(max(v[1] + g, 0), mx));
+                    R = lift<SHIFT>(min(max(v[2] + g, 0), mx));
+                    G = lift<SHIFT>(min(max(g, 0), mx));
+                    B = lift<SHIFT>(min(max(v[1] + g, 0), mx));
                 }
-                if (check_out == 0) G = B = R;
-                else if (check_out == 1) R = B = G;
-                else if (check_out == 2) R = G = B;
+                // grey preview of one component: one uniform test in the
+                // common case (check_out < 0), selects otherwise
+                if (CHECK && check_out >= 0) {
+                    const int g1 = check_out == 0 ? R : (check_out == 1 ? G : B);
+                    R = G = B = g1;
+                }
             } else {
                 R = G = B = 0;
             }
@@ -194,7 +267,7 @@ struct Emitter {
                     int sv;
                     if constexpr (FILT == 97) sv = (int)v[c];
                     else sv = v[c];
-                    t[c] = lift(min(max(sv + (1 << (bd - 1)), 0), mx));
+                    t[c] = lift<SHIFT>(min(max(sv + (1 << (bd - 1)), 0), mx));
                 }
             }
             merge_slots(t, fmt, check_out == 0, R, G, B, A);
@@ -257,19 +330,23 @@ struct Emitter {
         }
     }
 
-    // the two pixels of a lane, (row, ce) and (row, ce + 1)
+    // the two pixels of a lane, (row, ce) and (row, ce + 1).  The paired
+    // path is chosen per item (uniform), so a wave runs one copy of the
+    // colour code and masks only its stores.
     __device__ inline void put2(int row, int ce, const T *ve, const T *vo, bool e_ok, bool o_ok) const
     {
-        if (pair_fast && e_ok && o_ok) {
+        if (pair_fast && uniform_pairs) {
             if constexpr (OUTK == K_PLANE) {
                 const uint32_t off = ((uint32_t)(row - b0) * (uint32_t)pitch + (uint32_t)(ce - a0)) * 4u;
+                if (e_ok) {
 #pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    typedef T v2 __attribute__((ext_vector_type(2)));
-                    v2 pairv;
-                    pairv.x = ve[c];
-                    pairv.y = vo[c];
-                    *(__attribute__((address_space(1))) v2 *)at(dst[c], off) = pairv;
+                    for (int c = 0; c < NC; ++c) {
+                        typedef T v2 __attribute__((ext_vector_type(2)));
+                        v2 pairv;
+                        pairv.x = ve[c];
+                        pairv.y = vo[c];
+                        *(__attribute__((address_space(1))) v2 *)at(dst[c], off) = pairv;
+                    }
                 }
                 return;
             } else if constexpr (PL == PL_BGR8) {
@@ -278,18 +355,7 @@ struct Emitter {
                 int R0, G0, B0, A0, R1, G1, B1, A1;
                 colour(ve, R0, G0, B0, A0);
                 colour(vo, R1, G1, B1, A1);
-                const uint32_t off = ((uint32_t)(oy + row) * (uint32_t)pitch + (uint32_t)(ox + ce)) * 3u;
-                const bool mid = (off & 2) != 0;
-                const uint32_t lo4 = (uint32_t)(B0 & 0xff) | ((uint32_t)(G0 & 0xff) << 8) |
-                                     ((uint32_t)(R0 & 0xff) << 16) | ((uint32_t)(B1 & 0xff) << 24);
-                const uint32_t hi4 = (uint32_t)(R0 & 0xff) | ((uint32_t)(B1 & 0xff) << 8) |
-                                     ((uint32_t)(G1 & 0xff) << 16) | ((uint32_t)(R1 & 0xff) << 24);
-                const uint32_t sh_lo = (uint32_t)(B0 & 0xff) | ((uint32_t)(G0 & 0xff) << 8);
-                const uint32_t sh_hi = (uint32_t)(G1 & 0xff) | ((uint32_t)(R1 & 0xff) << 8);
-                const uint32_t dw_off = mid ? off + 2 : off;
-                const uint32_t sh_off = mid ? off : off + 4;
-                *(__attribute__((address_space(1))) uint32_t *)at(out, dw_off) = mid ? hi4 : lo4;
-                *(__attribute__((address_space(1))) uint16_t *)at(out, sh_off) = (uint16_t)(mid ? sh_lo : sh_hi);
+                store_pair_bgr8(row, ce, R0, G0, B0, R1, G1, B1, e_ok);
                 return;
             }
         }
@@ -381,13 +447,19 @@ __device__ __attribute__((always_inline)) inline void process_item(
     const int kout0 = K0 + seg * seg_pairs;
     const int kout1 = min(Klast, kout0 + seg_pairs - 1);
 
-    // raw words of pair k: [comp][top-low, top-high, bottom-low, bottom-high]
-    // levels of 8+ rows need at most one reflection for the rows a segment
-    // touches (2 row pairs of halo beyond each end)
-    const bool tall = (b1 - b0) >= 8;
+    // raw words of pair k: [comp][top-low, top-high, bottom-low, bottom-high];
+    // a segment fetches rows in [b0 - 5, b1 + 4]; for a level of two or
+    // more rows three single reflections bring every one of them into
+    // range (one for a level of 8 rows or more), equal to the periodic
+    // symmetric extension (reflect)
+    const int nrefl = (b1 - b0) >= 8 ? 1 : 3;
     auto reflect_row = [&](int r) {
-        if (tall) return r < b0 ? 2 * b0 - r : (r >= b1 ? 2 * (b1 - 1) - r : r);
-        return reflect(r, b0, b1);
+#pragma unroll 1
+        for (int i = 0; i < nrefl; ++i) {
+            r = r < b0 ? 2 * b0 - r : r;
+            r = r >= b1 ? 2 * (b1 - 1) - r : r;
+        }
+        return r;
     };
     auto fetch = [&](int k, uint32_t (&r)[NC][4]) {
         const int rre = reflect_row(2 * k), rro = reflect_row(2 * k + 1);
@@ -415,6 +487,8 @@ __device__ __attribute__((always_inline)) inline void process_item(
     em.init(J);
     const bool ce_ok = lane_out && ce >= a0 && ce < a1;
     const bool co_ok = lane_out && co >= a0 && co < a1;
+    em.set_lanes(ce_ok, co_ok);
+    const bool full = em.full_ok();
     auto emit_row = [&](int row, const T *ve, const T *vo) {
         if (row < b0 || row >= b1) return;
         em.put2(row, ce, ve, vo, ce_ok, co_ok);
@@ -484,8 +558,13 @@ __device__ __attribute__((always_inline)) inline void process_item(
         }
         const int ko = k - LAG;
         if (ko >= kout0 && ko <= kout1) {
-            emit_row(2 * ko, evn[0], evn[1]);
-            emit_row(2 * ko + 1, odd[0], odd[1]);
+            if (full && 2 * ko >= b0 && 2 * ko + 1 < b1) {
+                em.put2_full(2 * ko, ce, evn[0], evn[1], ce_ok);
+                em.put2_full(2 * ko + 1, ce, odd[0], odd[1], ce_ok);
+            } else {
+                emit_row(2 * ko, evn[0], evn[1]);
+                emit_row(2 * ko + 1, odd[0], odd[1]);
+            }
         }
     };
 
@@ -580,6 +659,7 @@ __device__ __attribute__((always_inline)) inline void process_item_small(const L
     em.init(J);
     const bool ce_ok = lane_out && ce >= a0 && ce < a1;
     const bool co_ok = lane_out && co >= a0 && co < a1;
+    em.set_lanes(ce_ok, co_ok);
     auto emit_row = [&](int row, const T *ve, const T *vo) {
         if (row < b0 || row >= b1) return;
         em.put2(row, ce, ve, vo, ce_ok, co_ok);
