@@ -154,7 +154,7 @@ struct Emitter {
             fmx = (float)mx;
             foff = (float)(1 << (bd - 1));
             check_out = J.check_out;
-            pair_fast = (PL == PL_BGR8) && !((uintptr_t)J.out & 3) && !((pitch * 3) & 3) && !(ox & 1);
+            pair_fast = (PL == PL_BGR8) && !((uintptr_t)J.out & 3) && !(ox & 1);
         }
     }
 
@@ -174,11 +174,13 @@ struct Emitter {
     }
 
     // put2_full applies (rows inside the level, paired lanes, no element
-    // shift, no grey check-out): the common case of an item, decided once
+    // shift, no grey check-out of a colour transform; a merge keeps its
+    // grey slot choice): the common case of an item, decided once
     __device__ inline bool full_ok() const
     {
         if constexpr (OUTK == K_PLANE) return pair_fast && uniform_pairs;
-        else if constexpr (PL == PL_BGR8) return pair_fast && uniform_pairs && ls == 0 && rs == 0 && check_out < 0;
+        else if constexpr (PL == PL_BGR8)
+            return pair_fast && uniform_pairs && ls == 0 && rs == 0 && (OUTK == K_MERGE || check_out < 0);
         else return false;
     }
 
@@ -205,12 +207,25 @@ struct Emitter {
         }
     }
 
-    // B0 G0 R0 B1 G1 R1 at an even pixel (a 2-aligned byte offset): one
-    // aligned dword and one aligned short
+    // B0 G0 R0 B1 G1 R1 of an even output column.  With an even row start
+    // (an even pitch, or an even row of an odd one) the byte offset is even:
+    // one aligned dword and one aligned short; otherwise three shorts at odd
+    // offsets.  The parity is the row's (ox + ce is even), so uniform.
     __device__ inline void store_pair_bgr8(int row, int ce, int R0, int G0, int B0, int R1, int G1, int B1,
                                            bool e_ok) const
     {
         const uint32_t off = ((uint32_t)(oy + row) * (uint32_t)pitch + (uint32_t)(ox + ce)) * 3u;
+        if (((uint32_t)(oy + row) * (uint32_t)pitch) & 1u) {
+            if (e_ok) {
+                *(__attribute__((address_space(1))) uint16_t *)at(out, off) =
+                    (uint16_t)((B0 & 0xff) | ((G0 & 0xff) << 8));
+                *(__attribute__((address_space(1))) uint16_t *)at(out, off + 2) =
+                    (uint16_t)((R0 & 0xff) | ((B1 & 0xff) << 8));
+                *(__attribute__((address_space(1))) uint16_t *)at(out, off + 4) =
+                    (uint16_t)((G1 & 0xff) | ((R1 & 0xff) << 8));
+            }
+            return;
+        }
         const bool mid = (off & 2) != 0;
         const uint32_t lo4 = (uint32_t)(B0 & 0xff) | ((uint32_t)(G0 & 0xff) << 8) |
                              ((uint32_t)(R0 & 0xff) << 16) | ((uint32_t)(B1 & 0xff) << 24);
