@@ -199,12 +199,13 @@ int BMI_batch_plan_destroy(BmiBatchPlan *plan);
 int BMI_batch_plan_error_flag(BmiBatchPlan *plan);
 
 /* Autotuner: build a plan for `tiles` with each candidate tuning, replay
-   its graph once untimed and `reps` times timed with HIP events on
-   `stream`, and
-   return the fastest in *best (its mean time per execute in *best_us).
-   candidates == NULL selects the built-in grid (segment rows 16..128 x
-   items per CU 4..16 x 2 or 4 waves per workgroup, step mode).  Runs the
-   real decode: outputs of the tiles are written. */
+   its graph once untimed, then time `reps` graph replays and `reps` direct
+   executions with HIP events on `stream`; a candidate scores the faster
+   mean.  Returns the best in *best (its mean time per execute in
+   *best_us).  candidates == NULL selects the built-in grid (segment rows
+   16..128 x items per CU 4..16 x 2 or 4 waves per workgroup x merged or
+   per-variant launches of mixed steps, modes 0 and 3).  Runs the real
+   decode: outputs of the tiles are written. */
 int BMI_autotune(const BmiTileDesc *tiles, int ntiles, const BmiTuning *candidates, int ncand,
                  int reps, StreamId stream, BmiTuning *best, float *best_us);
 

@@ -154,7 +154,9 @@ struct Emitter {
             fmx = (float)mx;
             foff = (float)(1 << (bd - 1));
             check_out = J.check_out;
-            pair_fast = (PL == PL_BGR8) && !((uintptr_t)J.out & 3) && !(ox & 1);
+            // 24-bit layout (fixed by the variant, or the merge's format)
+            const bool bgr8 = (PL == PL_BGR8) || (PL == PL_ANY && pixel_layout(fmt) == PL_BGR8);
+            pair_fast = bgr8 && !((uintptr_t)J.out & 3) && !(ox & 1);
         }
     }
 
@@ -179,7 +181,7 @@ struct Emitter {
     __device__ inline bool full_ok() const
     {
         if constexpr (OUTK == K_PLANE) return pair_fast && uniform_pairs;
-        else if constexpr (PL == PL_BGR8)
+        else if constexpr (PL == PL_BGR8 || PL == PL_ANY)
             return pair_fast && uniform_pairs && ls == 0 && rs == 0 && (OUTK == K_MERGE || check_out < 0);
         else return false;
     }
@@ -199,7 +201,7 @@ struct Emitter {
                     *(__attribute__((address_space(1))) v2 *)at(dst[c], off) = pairv;
                 }
             }
-        } else if constexpr (PL == PL_BGR8) {
+        } else if constexpr (PL == PL_BGR8 || PL == PL_ANY) {
             int R0, G0, B0, A0, R1, G1, B1, A1;
             colour<false, false>(ve, R0, G0, B0, A0);
             colour<false, false>(vo, R1, G1, B1, A1);
@@ -364,9 +366,8 @@ struct Emitter {
                     }
                 }
                 return;
-            } else if constexpr (PL == PL_BGR8) {
-                // 6 bytes B0 G0 R0 B1 G1 R1 at an even, 2-aligned offset:
-                // one aligned dword and one aligned short
+            } else if constexpr (PL == PL_BGR8 || PL == PL_ANY) {
+                // 6 bytes B0 G0 R0 B1 G1 R1 (pair_fast: a 24-bit layout)
                 int R0, G0, B0, A0, R1, G1, B1, A1;
                 colour(ve, R0, G0, B0, A0);
                 colour(vo, R1, G1, B1, A1);
@@ -1392,8 +1393,12 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
             // filters / components / outputs) becomes one launch of the
             // dispatching kernel (variant -1) unless the tuning asks for one
             // launch per variant (3); mode 2 merges every step.  One launch
-            // per step beats concurrent per-variant launches for mixed
-            // batches; a single-variant step keeps its specialised kernel.
+            // per step beats per-variant launches for mixed batches (256
+            // mixed tiles: 0.49 against 0.72 ms).  With two variants doing
+            // large levels the specialised kernels can win (256
+            // three-component 9/7 tiles: 0.55 against 0.59 ms), since the
+            // dispatching kernel carries every variant's code and the
+            // largest register budget; the autotuner tries both.
             {
                 int n0 = -1, nv = 0;
                 for (size_t q = 0; q < P->launches.size(); ++q)
@@ -1748,10 +1753,12 @@ extern "C" int BMI_autotune(const BmiTileDesc *tiles, int ntiles, const BmiTunin
     if (candidates && ncand > 0) {
         cand.assign(candidates, candidates + ncand);
     } else {
-        // default grid: segment height x items per CU x waves per group
+        // default grid: segment height x items per CU x waves per group x
+        // (merged or per-variant launches of steps that mix variants)
         for (int seg : {16, 32, 64, 128})
             for (int mw : {4, 8, 16})
-                for (int wpg : {2, 4}) cand.push_back(BmiTuning{seg, wpg, 0, 0, mw});
+                for (int wpg : {2, 4})
+                    for (int mode : {0, 3}) cand.push_back(BmiTuning{seg, wpg, 0, mode, mw});
     }
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t e0, e1;
@@ -1767,19 +1774,25 @@ extern "C" int BMI_autotune(const BmiTileDesc *tiles, int ntiles, const BmiTunin
         rc = BMI_batch_plan_create(&P, tiles, ntiles, &cand[i]);
         if (rc) break;
         // warm-up (captures the plan's graph and first-touches the
-        // scratch), then `reps` timed graph replays, the way plans run
+        // scratch), then `reps` timed graph replays and `reps` timed
+        // direct launches, the two ways plans run; a candidate scores the
+        // faster
         rc = BMI_batch_plan_execute_graph(P, stream);
-        if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = BMI_E_HIP;
-        for (int r = 0; r < reps && !rc; ++r) rc = BMI_batch_plan_execute_graph(P, stream);
-        if (!rc && hipEventRecord(e1, s) != hipSuccess) rc = BMI_E_HIP;
-        float ms = 0.0f;
-        if (!rc && (hipEventSynchronize(e1) != hipSuccess ||
-                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-            rc = BMI_E_HIP;
+        float us = 0.0f;
+        for (int way = 0; way < 2 && !rc; ++way) {
+            if (hipEventRecord(e0, s) != hipSuccess) rc = BMI_E_HIP;
+            for (int r = 0; r < reps && !rc; ++r)
+                rc = way ? BMI_batch_plan_execute(P, stream) : BMI_batch_plan_execute_graph(P, stream);
+            if (!rc && hipEventRecord(e1, s) != hipSuccess) rc = BMI_E_HIP;
+            float ms = 0.0f;
+            if (!rc && (hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = BMI_E_HIP;
+            if (!rc && (way == 0 || ms * 1000.0f / reps < us)) us = ms * 1000.0f / reps;
+        }
         if (!rc && BMI_batch_plan_error_flag(P) != 0) rc = BMI_E_HIP;
         BMI_batch_plan_destroy(P);
         if (rc) break;
-        const float us = ms * 1000.0f / reps;
         if (best_i < 0 || us < best_t) {
             best_t = us;
             best_i = (int)i;
