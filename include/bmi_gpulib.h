@@ -157,7 +157,8 @@ typedef struct BmiTileDesc {
 /* Launch shape of the fused kernel (0 = library default / autotuned). */
 typedef struct BmiTuning {
     int seg_rows;        /* maximum output rows per work item (even) */
-    int waves_per_group; /* 1..16 wave64s per workgroup */
+    int waves_per_group; /* 1..4 wave64s per workgroup (0: 1 for launches of
+                            under four items per SIMD, else 4) */
     int groups_per_cu;   /* persistent workgroups per CU */
     int persistent;      /* 0 (default) = one launch per level step (a step
                             mixing kernel variants runs them in one
@@ -203,7 +204,7 @@ int BMI_batch_plan_error_flag(BmiBatchPlan *plan);
    executions with HIP events on `stream`; a candidate scores the faster
    mean.  Returns the best in *best (its mean time per execute in
    *best_us).  candidates == NULL selects the built-in grid (segment rows
-   16..128 x items per CU 4..16 x 2 or 4 waves per workgroup x merged or
+   16..128 x items per CU 4..16 x 2, 4 or per-launch waves per workgroup x merged or
    per-variant launches of mixed steps, modes 0 and 3).  Runs the real
    decode: outputs of the tiles are written. */
 int BMI_autotune(const BmiTileDesc *tiles, int ntiles, const BmiTuning *candidates, int ncand,

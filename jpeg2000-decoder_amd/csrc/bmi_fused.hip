@@ -979,6 +979,7 @@ struct BmiBatchPlan {
     std::vector<int> step_first, step_count;  // level-step mode
     struct Launch {
         int first, count, variant, step;
+        int wpg = 0;  // waves per workgroup of this launch
     };
     std::vector<Launch> launches;  // level-step mode: per step and variant
     std::vector<int> job_tile;                // tile of each job
@@ -1102,7 +1103,8 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
     P->tiles.assign(tiles, tiles + ntiles);
     P->tuning.seg_rows = default_seg_rows(tuning);
     if (P->tuning.seg_rows & 1) P->tuning.seg_rows += 1;
-    P->tuning.waves_per_group = (tuning && tuning->waves_per_group > 0) ? tuning->waves_per_group : 4;
+    const bool wpg_auto = !(tuning && tuning->waves_per_group > 0);
+    P->tuning.waves_per_group = wpg_auto ? 4 : tuning->waves_per_group;
     P->tuning.groups_per_cu = (tuning && tuning->groups_per_cu > 0) ? tuning->groups_per_cu : 8;
     P->tuning.persistent = tuning ? tuning->persistent : 0;
     P->tuning.min_waves_per_cu = (tuning && tuning->min_waves_per_cu > 0) ? tuning->min_waves_per_cu : 4;
@@ -1417,6 +1419,15 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
         P->step_count[step] = (int)P->items.size() - P->step_first[step];
     }
 
+    // Waves per workgroup of each step launch.  A launch of a specialised
+    // kernel with fewer than four waves' worth of items per SIMD runs one
+    // wave per workgroup: the dispatcher then spreads the items over the
+    // CUs in single waves, and no CU holds a whole extra workgroup while
+    // others idle at the end (4096x2160x3 final level: 35.4 -> 34.6 us).
+    // Larger launches and the dispatching kernel keep the tuning's value.
+    for (auto &l : P->launches)
+        l.wpg = (wpg_auto && l.variant >= 0 && l.count < 4 * 4 * P->num_cus) ? 1 : P->tuning.waves_per_group;
+
     const size_t jb = sizeof(LevelJob) * P->jobs.size();
     const size_t ib = sizeof(WorkItem) * P->items.size();
     const size_t cb = sizeof(int) * (P->jobs.size() + 3);
@@ -1530,7 +1541,7 @@ extern "C" int BMI_batch_plan_create(BmiBatchPlan **out_plan, const BmiTileDesc 
 
 static int launch_one(const BmiBatchPlan *P, const BmiBatchPlan::Launch &l, hipStream_t s)
 {
-    const int wpg = P->tuning.waves_per_group;
+    const int wpg = l.wpg > 0 ? l.wpg : P->tuning.waves_per_group;
     const int xcd_remap = 1;
     const dim3 grid((l.count + wpg - 1) / wpg), block(64 * wpg);
     if (l.variant < 0) {
@@ -1757,7 +1768,7 @@ extern "C" int BMI_autotune(const BmiTileDesc *tiles, int ntiles, const BmiTunin
         // (merged or per-variant launches of steps that mix variants)
         for (int seg : {16, 32, 64, 128})
             for (int mw : {4, 8, 16})
-                for (int wpg : {2, 4})
+                for (int wpg : {0, 2, 4})   // 0: chosen per launch
                     for (int mode : {0, 3}) cand.push_back(BmiTuning{seg, wpg, 0, mode, mw});
     }
     hipStream_t s = (hipStream_t)stream;
