@@ -259,6 +259,7 @@ def t1_leg(pkg, wl, torch, dev, stream, steps, warmup):
         blocks.append(b)
     pkg.t1_check_blocks(blocks)
     arr = (pkg.BmiT1Block * len(blocks))(*blocks)
+    pkg.t1_order_blocks(arr)   # longest code-block data first (BMI_t1_order_blocks)
     dblocks = torch.from_numpy(np.frombuffer(bytes(arr), dtype=np.uint8).copy()).to(dev)
     ddata = torch.from_numpy(np.frombuffer(img["data"], dtype=np.uint8).copy()).to(dev)
     out = torch.zeros((H, W * 3), dtype=torch.uint8, device=dev)

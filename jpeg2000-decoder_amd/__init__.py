@@ -262,6 +262,7 @@ def lib() -> ctypes.CDLL:
     L.BMI_t1_decode_blocks.restype = i
     L.BMI_t1_check_blocks.argtypes = [ctypes.POINTER(BmiT1Block), i, ctypes.POINTER(i)]
     L.BMI_t1_num_segments.argtypes = [i, i]
+    L.BMI_t1_order_blocks.argtypes = [ctypes.POINTER(BmiT1Block), i]
     L.BMI_t1_check_blocks.restype = i
     L.BMI_plain_decode.argtypes = [ctypes.POINTER(BmiTileDesc), vp, ctypes.c_size_t, vp]
     L.BMI_plain_decode.restype = i
@@ -483,6 +484,12 @@ def t1_check_blocks(blocks) -> None:
         raise BmiError(f"BMI_t1_check_blocks: block {bad.value}: {lib().BMI_error_string(rc).decode()}")
 
 
+def t1_order_blocks(blocks) -> None:
+    """BMI_t1_order_blocks on a ctypes BmiT1Block array (in place): longest
+    code-block data first."""
+    check(lib().BMI_t1_order_blocks(blocks, len(blocks)), "BMI_t1_order_blocks")
+
+
 def t1_decode_blocks(blocks_dev_ptr: int, nblocks: int, data_dev_ptr: int, stream: int = 0) -> None:
     """BMI_t1_decode_blocks on device block table and code-stream buffer."""
     check(lib().BMI_t1_decode_blocks(blocks_dev_ptr, nblocks, data_dev_ptr, stream or None),
@@ -680,6 +687,11 @@ class CodestreamDecoder:
         self.seg_bytes = seg_base
         self.d_segs = torch.from_numpy(np.frombuffer(seg_all, np.uint8).copy()).to(device)
         tab = b"".join(tables)
+        if tab:
+            # longest code-block data first: a shorter tail of the launch
+            arr = (BmiT1Block * (len(tab) // ctypes.sizeof(BmiT1Block))).from_buffer_copy(tab)
+            t1_order_blocks(arr)
+            tab = bytes(arr)
         self.d_blocks = torch.from_numpy(np.frombuffer(tab or b"\0", np.uint8).copy()).to(device)
         self.descs = descs
         # tiles without decomposition levels have no synthesis: the plain

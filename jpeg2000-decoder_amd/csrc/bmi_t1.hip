@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "bmi_device.h"
 #include "bmi_internal.h"
 #include "bmi_t1.h"
@@ -734,6 +736,14 @@ extern "C" int BMI_t1_check_blocks(const BmiT1Block *blocks, int nblocks, int *b
             return bmi_fail(rc, "t1_check_blocks: block %d rejected", i);
         }
     }
+    return BMI_OK;
+}
+
+extern "C" int BMI_t1_order_blocks(BmiT1Block *blocks, int nblocks)
+{
+    if (nblocks < 0 || (nblocks > 0 && !blocks)) return bmi_fail(BMI_E_INVALID, "t1_order_blocks: bad args");
+    std::stable_sort(blocks, blocks + nblocks,
+                     [](const BmiT1Block &a, const BmiT1Block &b) { return a.len > b.len; });
     return BMI_OK;
 }
 

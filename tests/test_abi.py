@@ -72,3 +72,19 @@ def test_plan_rejects_bad_descriptors_without_gpu(pkg):
                                  out_ptr=1, out_pitch_px=8)
     with pytest.raises(pkg.BmiError):
         pkg.BatchPlan([rgb_one_comp])
+
+
+def test_t1_order_blocks_longest_first(pkg):
+    """BMI_t1_order_blocks: a stable permutation of the table, longest data
+    first (host only)."""
+    import ctypes
+    lens = [5, 90, 0, 90, 33, 7, 33, 1]
+    arr = (pkg.BmiT1Block * len(lens))()
+    for i, n in enumerate(lens):
+        arr[i].len = n
+        arr[i].data_offset = i          # identifies the entry
+    pkg.t1_order_blocks(arr)
+    got = [(arr[i].len, arr[i].data_offset) for i in range(len(lens))]
+    assert got == sorted(((n, i) for i, n in enumerate(lens)), key=lambda x: -x[0])
+    assert pkg.lib().BMI_t1_order_blocks(None, 0) == 0
+    assert pkg.lib().BMI_t1_order_blocks(None, 3) != 0
