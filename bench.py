@@ -3,12 +3,14 @@
 Workload (one step): decode one 4096x2160 3-component image whose
 coefficients sit in HBM -- 5-level irreversible 9/7 inverse DWT on float32
 subbands (LL ~ N(128,40), detail Laplace(4)*2^-level), inverse ICT, level
-shift, clamp, packed 24-bit BGR -- in one persistent launch of the fused
-engine, replayed from a hipGraph.  Prints one JSON line.
+shift, clamp, packed 24-bit BGR -- with the fused engine, one launch per
+level step, timed with direct launches and with a hipGraph replay (the
+faster is reported).  Prints one JSON line.
 
 Also measured, beside `value`: the plain reference-shaped GPU path (the
 drop-in BMI_comp_idwt per component + BMI_tile_ICT, 2 launches per level),
-the dominant kernel against HBM roofline, and the CPU oracle on the host.
+the dominant kernel against HBM roofline, the 256-tile batch, the request
+server, tier 1 and the file-to-pixels path, and the CPU oracle on the host.
 """
 from __future__ import annotations
 
