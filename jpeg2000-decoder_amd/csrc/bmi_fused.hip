@@ -1346,23 +1346,65 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                 push_job(J, -1);
             }
         }
-        // segment height: start at the tuned maximum and halve it for all
-        // jobs of the step until the step has enough items to fill the GPU
-        // (small coarse levels are latency-bound, not bandwidth-bound)
+        // Segment height of the step: the same number of row pairs for every
+        // job, chosen by a cost model of the launch.  When the step's items
+        // at the tuned height fit on the GPU at once (at most four waves per
+        // SIMD), they all run side by side and the step lasts about as long
+        // as its busiest SIMD: rounds = ceil(items / SIMDs) waves, each
+        // covering its rows plus the vertical halo it re-reads (8 rows for
+        // 9/7, 4 for 5/3).  The model then picks, up to twice the tuned
+        // height, the one with the fewest rounds x (rows + halo) among those
+        // giving the step at least min_waves_per_cu items per CU (small
+        // coarse levels are latency-bound, not bandwidth-bound); ties go to
+        // the taller segment.  4096x2160 final level: 38-row segments (1995
+        // items, two per SIMD) run in 31.0 us, 32-row ones (2380 items,
+        // some SIMDs three) in 34.4 us.  Larger steps, which the dispatcher
+        // refills as waves finish, search only up to the tuned height (256
+        // mixed tiles: 0.49 ms at 32 rows, 0.50 ms with 64 allowed).
+        // Thumbnail jobs keep their one-pair items.
         {
-            const int target = P->num_cus * P->tuning.min_waves_per_cu;
-            for (;;) {
-                long total = 0;
-                bool can = false;
-                for (int j = step_job0; j < (int)P->jobs.size(); ++j) {
-                    total += P->jobs[j].n_items;
-                    can = can || P->jobs[j].seg_pairs > 1;
+            const long target = (long)P->num_cus * P->tuning.min_waves_per_cu;
+            const long simds = 4L * P->num_cus;
+            long fixed = 0;
+            bool any97 = false, any = false;
+            for (int j = step_job0; j < (int)P->jobs.size(); ++j) {
+                const LevelJob &J = P->jobs[j];
+                if (J.seg_pairs <= 1) {
+                    fixed += J.n_items;
+                } else {
+                    any = true;
+                    any97 = any97 || J.filt == 97;
                 }
-                if (total >= target || !can) break;
+            }
+            if (any) {
+                const int halo = any97 ? 8 : 4;
+                auto items_at = [&](int sp) {
+                    long total = fixed;
+                    for (int j = step_job0; j < (int)P->jobs.size(); ++j) {
+                        const LevelJob &J = P->jobs[j];
+                        if (J.seg_pairs <= 1) continue;
+                        const int nrows = J.b1 - (J.b0 & ~1);
+                        total += (long)J.nstrips * ((nrows + 2 * sp - 1) / (2 * sp));
+                    }
+                    return total;
+                };
+                const int tuned = P->tuning.seg_rows / 2;
+                const int top = items_at(tuned) <= 4 * simds ? 2 * tuned : tuned;
+                int best = 1;
+                long best_cost = -1;
+                for (int sp = top; sp >= 1; --sp) {
+                    const long total = items_at(sp);
+                    if (total < target && sp > 1) continue;
+                    const long cost = ((total + simds - 1) / simds) * (2 * sp + halo);
+                    if (best_cost < 0 || cost < best_cost) {
+                        best_cost = cost;
+                        best = sp;
+                    }
+                }
                 for (int j = step_job0; j < (int)P->jobs.size(); ++j) {
                     LevelJob &J = P->jobs[j];
                     if (J.seg_pairs <= 1) continue;
-                    J.seg_pairs >>= 1;
+                    J.seg_pairs = best;
                     const int nrows = J.b1 - (J.b0 & ~1);
                     J.nsegs = (nrows + 2 * J.seg_pairs - 1) / (2 * J.seg_pairs);
                     J.n_items = J.nstrips * J.nsegs;
