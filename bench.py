@@ -422,8 +422,8 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build decodes on a single MI355X "
-              "(tiles and images shard across GPUs with no exchange; see DESIGN.md). "
+        print("bench.py measures one GPU only: the library decodes on a single MI355X "
+              "(tiles and images shard across GPUs with no exchange between them). "
               "Run with --gpus 1.", file=sys.stderr)
         return 2
 
