@@ -1350,9 +1350,9 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
         // job, chosen by a cost model of the launch.  When the step's items
         // at the tuned height fit on the GPU at once (at most four waves per
         // SIMD), they all run side by side and the step lasts about as long
-        // as its busiest SIMD: rounds = ceil(items / SIMDs) waves, each
-        // covering its rows plus the vertical halo it re-reads (8 rows for
-        // 9/7, 4 for 5/3).  The model then picks, up to twice the tuned
+        // as its busiest SIMD: rounds = ceil(items / SIMDs) waves (at least
+        // two), each covering its rows plus the vertical halo it re-reads (8
+        // rows for 9/7, 4 for 5/3).  The model then picks, up to twice the tuned
         // height, the one with the fewest rounds x (rows + halo) among those
         // giving the step at least min_waves_per_cu items per CU (small
         // coarse levels are latency-bound, not bandwidth-bound); ties go to
@@ -1395,7 +1395,11 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                 for (int sp = top; sp >= 1; --sp) {
                     const long total = items_at(sp);
                     if (total < target && sp > 1) continue;
-                    const long cost = ((total + simds - 1) / simds) * (2 * sp + halo);
+                    // one wave per SIMD leaves memory latency unhidden: it
+                    // costs about as much as two (final level above in
+                    // 76-row segments, 1015 items: 44 us against 31 us)
+                    const long rounds = std::max(2L, (total + simds - 1) / simds);
+                    const long cost = rounds * (2 * sp + halo);
                     if (best_cost < 0 || cost < best_cost) {
                         best_cost = cost;
                         best = sp;
