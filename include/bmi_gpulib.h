@@ -156,7 +156,11 @@ typedef struct BmiTileDesc {
 
 /* Launch shape of the fused kernel (0 = library default / autotuned). */
 typedef struct BmiTuning {
-    int seg_rows;        /* maximum output rows per work item (even) */
+    int seg_rows;        /* tuned output rows per work item (even, default
+                            32).  Each level step takes, from a cost model
+                            of its waves per SIMD, a height up to this one,
+                            or up to twice it when all the step's items fit
+                            on the GPU at once */
     int waves_per_group; /* 1..4 wave64s per workgroup (0: 1 for launches of
                             under four items per SIMD, else 4) */
     int groups_per_cu;   /* persistent workgroups per CU */
@@ -166,8 +170,9 @@ typedef struct BmiTuning {
                             with a device work queue; 2 = every step as one
                             dispatching launch; 3 = one launch per step and
                             variant */
-    int min_waves_per_cu; /* segment heights shrink until each level step
-                             offers this many items per CU (default 4) */
+    int min_waves_per_cu; /* segment heights are chosen among those giving
+                             each level step this many items per CU, or
+                             the smallest height (default 4) */
 } BmiTuning;
 
 typedef struct BmiBatchPlan BmiBatchPlan;
