@@ -1367,6 +1367,7 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
             const long simds = 4L * P->num_cus;
             long fixed = 0;
             bool any97 = false, any = false;
+            int max_pairs = 1;  // row pairs of the tallest job: no taller segment helps
             for (int j = step_job0; j < (int)P->jobs.size(); ++j) {
                 const LevelJob &J = P->jobs[j];
                 if (J.seg_pairs <= 1) {
@@ -1374,6 +1375,7 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                 } else {
                     any = true;
                     any97 = any97 || J.filt == 97;
+                    max_pairs = std::max(max_pairs, (J.b1 - (J.b0 & ~1) + 1) / 2);
                 }
             }
             if (any) {
@@ -1388,8 +1390,8 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                     }
                     return total;
                 };
-                const int tuned = P->tuning.seg_rows / 2;
-                const int top = items_at(tuned) <= 4 * simds ? 2 * tuned : tuned;
+                const int tuned = std::min(P->tuning.seg_rows / 2, max_pairs);
+                const int top = std::min(max_pairs, items_at(tuned) <= 4 * simds ? 2 * tuned : tuned);
                 int best = 1;
                 long best_cost = -1;
                 for (int sp = top; sp >= 1; --sp) {
