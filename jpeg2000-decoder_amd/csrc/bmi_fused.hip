@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "bmi_device.h"
@@ -510,14 +511,24 @@ __device__ __attribute__((always_inline)) inline void process_item(
         em.put2(row, ce, ve, vo, ce_ok, co_ok);
     };
 
+    // 9/7 segments alternate direction: odd segments run bottom-up.  The
+    // halo rows two vertically neighbouring segments both read are then
+    // read at the same time by both (at the end of each, or at the start
+    // of each), so the second read is served by L2 instead of HBM.  Every
+    // lifting step keeps its operands and their order, so the results are
+    // the same bit for bit in either direction.
+    const bool rev = (FILT == 97) && (seg & 1);
+
     // vertical pipeline state per component and column (0 = ce, 1 = co)
-    // 9/7: s0 = H_{k-1}, s1 = e3_{k-1}, s2 = o4_{k-2}, s3 = e5_{k-2}
+    // 9/7 top-down:  s0 = H_{k-1}, s1 = e3_{k-1}, s2 = o4_{k-2}, s3 = e5_{k-2}
+    // 9/7 bottom-up: s0 = L_{k+1}, s1 = H_{k+1}, s2 = e3_{k+2}, s3 = o4_{k+2},
+    //                s4 = e5_{k+3}
     // 5/3: s0 = H_{k-1}, s1 = e1_{k-1}, s2 = L_{k-1}, s3 = H_{k-2}
-    T s0[NC][2], s1[NC][2], s2[NC][2], s3[NC][2];
+    T s0[NC][2], s1[NC][2], s2[NC][2], s3[NC][2], s4[NC][2];
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int e = 0; e < 2; ++e) s0[c][e] = s1[c][e] = s2[c][e] = s3[c][e] = (T)0;
+        for (int e = 0; e < 2; ++e) s0[c][e] = s1[c][e] = s2[c][e] = s3[c][e] = s4[c][e] = (T)0;
 
     const int nh = b1 - b0;
     // iteration at which the reference's VDWT_R53_LINES tail applies:
@@ -529,7 +540,8 @@ __device__ __attribute__((always_inline)) inline void process_item(
     const int ke = kout1 + LAG;
 
     // one pipeline step with pair k (samples already converted)
-    auto consume = [&](int k, T (&smp)[NC][4]) {
+    auto consume = [&](auto dir, int k, T (&smp)[NC][4]) {
+        constexpr bool REV = decltype(dir)::value;
         T evn[2][NC], odd[2][NC];  // [column][component]
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
@@ -540,16 +552,33 @@ __device__ __attribute__((always_inline)) inline void process_item(
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 if constexpr (FILT == 97) {
-                    const float e3 = lift97(Lk[e], kDelta, s0[c][e], Hk[e]);
-                    const float o4 = lift97(s0[c][e], kGamma, s1[c][e], e3);
-                    const float e5 = lift97(s1[c][e], kBeta, s2[c][e], o4);
-                    const float o6 = lift97(s2[c][e], kAlpha, s3[c][e], e5);
-                    evn[e][c] = s3[c][e];
-                    odd[e][c] = o6;
-                    s0[c][e] = Hk[e];
-                    s1[c][e] = e3;
-                    s2[c][e] = o4;
-                    s3[c][e] = e5;
+                    if constexpr (!REV) {
+                        // pair k in, pair k - 2 out
+                        const float e3 = lift97(Lk[e], kDelta, s0[c][e], Hk[e]);
+                        const float o4 = lift97(s0[c][e], kGamma, s1[c][e], e3);
+                        const float e5 = lift97(s1[c][e], kBeta, s2[c][e], o4);
+                        const float o6 = lift97(s2[c][e], kAlpha, s3[c][e], e5);
+                        evn[e][c] = s3[c][e];
+                        odd[e][c] = o6;
+                        s0[c][e] = Hk[e];
+                        s1[c][e] = e3;
+                        s2[c][e] = o4;
+                        s3[c][e] = e5;
+                    } else {
+                        // pair k in, pair k + 2 out; the same steps with the
+                        // upper neighbour first, as above
+                        const float e3 = lift97(s0[c][e], kDelta, Hk[e], s1[c][e]);  // e3_{k+1}
+                        const float o4 = lift97(s1[c][e], kGamma, e3, s2[c][e]);     // o4_{k+1}
+                        const float e5 = lift97(s2[c][e], kBeta, o4, s3[c][e]);      // e5_{k+2}
+                        const float o6 = lift97(s3[c][e], kAlpha, e5, s4[c][e]);     // o6_{k+2}
+                        evn[e][c] = e5;
+                        odd[e][c] = o6;
+                        s0[c][e] = Lk[e];
+                        s1[c][e] = Hk[e];
+                        s2[c][e] = e3;
+                        s3[c][e] = o4;
+                        s4[c][e] = e5;
+                    }
                 } else {
                     int e1 = Lk[e] - ((s0[c][e] + Hk[e] + 2) >> 2);
                     int o2 = s0[c][e] + ((s1[c][e] + e1) >> 1);
@@ -572,7 +601,7 @@ __device__ __attribute__((always_inline)) inline void process_item(
                 }
             }
         }
-        const int ko = k - LAG;
+        const int ko = REV ? k + 2 : k - LAG;
         if (ko >= kout0 && ko <= kout1) {
             if (full && 2 * ko >= b0 && 2 * ko + 1 < b1) {
                 em.put2_full(2 * ko, ce, evn[0], evn[1], ce_ok);
@@ -585,31 +614,40 @@ __device__ __attribute__((always_inline)) inline void process_item(
     };
 
     // software pipeline, D pairs of loads in flight; loads are issued
-    // unconditionally (rows past the segment end are clamped to it and the
-    // pipeline steps that use them emit nothing)
-    uint32_t ring[D][NC][4];
+    // unconditionally (pairs past the range [ks, ke] are clamped to it and
+    // the pipeline steps that use them emit nothing); bottom-up segments
+    // walk the same pairs from ke down to ks
+    // (one copy of the loop per direction, chosen once per item)
+    auto run = [&](auto dir) {
+        constexpr bool REV = decltype(dir)::value;
+        uint32_t ring[D][NC][4];
 #pragma unroll
-    for (int d = 0; d < D; ++d) fetch(min(ks + d, ke), ring[d]);
-    const int rounds = (ke - ks + D) / D;
-    for (int it = 0; it < rounds; ++it) {
+        for (int d = 0; d < D; ++d) fetch(REV ? max(ke - d, ks) : min(ks + d, ke), ring[d]);
+        const int rounds = (ke - ks + D) / D;
+        for (int it = 0; it < rounds; ++it) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const int k = ks + it * D + d;
-            // convert slot d (its loads were issued D pairs ago), then reuse
-            // its registers for pair k + D before doing the arithmetic
-            T smp[NC][4];
+            for (int d = 0; d < D; ++d) {
+                const int k = REV ? ke - (it * D + d) : ks + it * D + d;
+                // convert slot d (its loads were issued D pairs ago), then reuse
+                // its registers for pair k + D before doing the arithmetic
+                T smp[NC][4];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                smp[c][0] = to_sample<FILT, SK_LL>(ring[d][c][0], ll_float, gn[c][0]);
-                smp[c][1] = to_sample<FILT, SK_DET>(ring[d][c][1], det_float, gn[c][1]);
-                smp[c][2] = to_sample<FILT, SK_DET>(ring[d][c][2], det_float, gn[c][2]);
-                smp[c][3] = to_sample<FILT, SK_DET>(ring[d][c][3], det_float, gn[c][3]);
+                for (int c = 0; c < NC; ++c) {
+                    smp[c][0] = to_sample<FILT, SK_LL>(ring[d][c][0], ll_float, gn[c][0]);
+                    smp[c][1] = to_sample<FILT, SK_DET>(ring[d][c][1], det_float, gn[c][1]);
+                    smp[c][2] = to_sample<FILT, SK_DET>(ring[d][c][2], det_float, gn[c][2]);
+                    smp[c][3] = to_sample<FILT, SK_DET>(ring[d][c][3], det_float, gn[c][3]);
+                }
+                fetch(REV ? max(k - D, ks) : min(k + D, ke), ring[d]);
+                consume(dir, k, smp);
+                __builtin_amdgcn_sched_barrier(0);
             }
-            fetch(min(k + D, ke), ring[d]);
-            consume(k, smp);
-            __builtin_amdgcn_sched_barrier(0);
         }
-    }
+    };
+    if (rev)
+        run(std::true_type{});
+    else
+        run(std::false_type{});
 }
 
 template <int FILT, int NC, int OUTK, int PL>
