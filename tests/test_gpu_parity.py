@@ -144,3 +144,20 @@ def test_fused_launch_shapes_agree(pkg, workloads, orc, seg_rows, persistent):
     t = pkg.BmiTuning(seg_rows, 4, 4, persistent, 0)
     _, got = run_fused(pkg, planes, tuning=t, filt="97", levels=L, out_format=2)
     assert np.array_equal(got, exp), diff_report(got, exp)
+
+
+@pytest.mark.parametrize("seg_rows", [2, 4, 6, 10, 38])
+@pytest.mark.parametrize("y0", [0, 1, 6])
+def test_alternating_segment_directions_match_oracle(pkg, workloads, orc, seg_rows, y0):
+    """9/7 level steps run odd segments bottom-up and even ones top-down;
+    float planes for several segment heights and row origins (an odd origin
+    starts the tile on a high-pass row, the first and last segments cross the
+    symmetric extension) equal the oracle's bit for bit."""
+    w, h, L = 157, 203, 4
+    planes = workloads.tile_planes(w, h, L, "97", 2, seed=seg_rows * 10 + y0, y0=y0)
+    exp, _ = orc.decode_tile(planes, levels=L, filt="97", out_format=7, y0=y0)
+    for persistent in (0, 1):
+        t = pkg.BmiTuning(seg_rows, 0, 0, persistent, 1)
+        got, _ = run_fused(pkg, planes, tuning=t, filt="97", levels=L, out_format=pkg.FMT_NONE,
+                           y0=y0)
+        _check_planes(got, exp, "97", f"seg {seg_rows} y0 {y0} persistent {persistent}")
