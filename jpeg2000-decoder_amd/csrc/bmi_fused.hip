@@ -202,12 +202,57 @@ struct Emitter {
                     *(__attribute__((address_space(1))) v2 *)at(dst[c], off) = pairv;
                 }
             }
+        } else if constexpr (FILT == 97 && OUTK == K_RGB && NC >= 3 && PL == PL_BGR8) {
+            // 8-bit samples with no shift (full_ok): a channel is
+            // (int)clamp(x, 0, 255), i.e. the clamped truncation, which
+            // v_cvt_pk_u8_f32 of trunc(x) computes and packs in one step
+            float r0, g0, b0, r1, g1, b1;
+            ict97(ve, r0, g0, b0);
+            ict97(vo, r1, g1, b1);
+            auto u8 = [](float x, uint32_t k, uint32_t acc) {
+                return __builtin_amdgcn_cvt_pk_u8_f32(__builtin_truncf(x), k, acc);
+            };
+            // w0 = B0 G0 R0 B1, w1 = G1 R1 (bytes, lowest first)
+            const uint32_t w0 = u8(b1, 3, u8(r0, 2, u8(g0, 1, u8(b0, 0, 0u))));
+            const uint32_t w1 = u8(r1, 1, u8(g1, 0, 0u));
+            store_pair_words(row, ce, w0, w1, e_ok);
         } else if constexpr (PL == PL_BGR8 || PL == PL_ANY) {
             int R0, G0, B0, A0, R1, G1, B1, A1;
             colour<false, false>(ve, R0, G0, B0, A0);
             colour<false, false>(vo, R1, G1, B1, A1);
             store_pair_bgr8(row, ce, R0, G0, B0, R1, G1, B1, e_ok);
         }
+    }
+
+    // store_pair_bgr8 of the six bytes already packed: w0 = B0 G0 R0 B1,
+    // w1 = G1 R1 (lowest byte first)
+    __device__ inline void store_pair_words(int row, int ce, uint32_t w0, uint32_t w1, bool e_ok) const
+    {
+        const uint32_t off = ((uint32_t)(oy + row) * (uint32_t)pitch + (uint32_t)(ox + ce)) * 3u;
+        if (((uint32_t)(oy + row) * (uint32_t)pitch) & 1u) {
+            if (e_ok) {
+                *(__attribute__((address_space(1))) uint16_t *)at(out, off) = (uint16_t)w0;
+                *(__attribute__((address_space(1))) uint16_t *)at(out, off + 2) = (uint16_t)(w0 >> 16);
+                *(__attribute__((address_space(1))) uint16_t *)at(out, off + 4) = (uint16_t)w1;
+            }
+            return;
+        }
+        const bool mid = (off & 2) != 0;
+        const uint32_t hi4 = __builtin_amdgcn_alignbit(w1, w0, 16);  // R0 B1 G1 R1
+        const uint32_t dw_off = mid ? off + 2 : off;
+        const uint32_t sh_off = mid ? off : off + 4;
+        if (e_ok) {
+            *(__attribute__((address_space(1))) uint32_t *)at(out, dw_off) = mid ? hi4 : w0;
+            *(__attribute__((address_space(1))) uint16_t *)at(out, sh_off) = (uint16_t)(mid ? w0 : w1);
+        }
+    }
+
+    // RICT_97_FLOAT of one pixel before the clamp (bmi_cpu_dwt_decoder.cpp:3570-3841)
+    __device__ inline void ict97(const T *v, float &r, float &g, float &b) const
+    {
+        r = __fadd_rn(__fadd_rn(v[0], __fmul_rn(kCrR, v[2])), foff);
+        g = __fadd_rn(__fadd_rn(__fadd_rn(v[0], __fmul_rn(kCbG, v[1])), __fmul_rn(kCrG, v[2])), foff);
+        b = __fadd_rn(__fadd_rn(v[0], __fmul_rn(kCbB, v[1])), foff);
     }
 
     // B0 G0 R0 B1 G1 R1 of an even output column.  With an even row start
