@@ -146,6 +146,19 @@ def test_fused_launch_shapes_agree(pkg, workloads, orc, seg_rows, persistent):
     assert np.array_equal(got, exp), diff_report(got, exp)
 
 
+@pytest.mark.parametrize("w", [300, 301])
+def test_ict_saturation_matches_oracle(pkg, workloads, orc, w):
+    """9/7 + ICT to 8-bit BGR where most pixels clamp at 0 or 255 (the
+    packed-conversion store path; an odd width puts odd rows at odd byte
+    offsets), bit for bit against the oracle."""
+    h, L = 200, 3
+    planes = workloads.tile_planes(w, h, L, "97", 3, seed=77, ll97=(-300.0, 500.0))
+    _, exp = orc.decode_tile(planes, levels=L, filt="97", out_format=2)
+    assert (exp == 0).mean() > 0.3 and (exp == 255).mean() > 0.1 and ((exp > 0) & (exp < 255)).mean() > 0.02
+    _, got = run_fused(pkg, planes, filt="97", levels=L, out_format=2)
+    assert np.array_equal(got, exp), diff_report(got, exp)
+
+
 @pytest.mark.parametrize("seg_rows", [2, 4, 6, 10, 38])
 @pytest.mark.parametrize("y0", [0, 1, 6])
 def test_alternating_segment_directions_match_oracle(pkg, workloads, orc, seg_rows, y0):
