@@ -202,8 +202,9 @@ struct Emitter {
                     *(__attribute__((address_space(1))) v2 *)at(dst[c], off) = pairv;
                 }
             }
-        } else if constexpr (FILT == 97 && OUTK == K_RGB && NC >= 3 && PL == PL_BGR8) {
-            // 8-bit samples with no shift (full_ok): a channel is
+        } else if constexpr (FILT == 97 && OUTK == K_RGB && NC >= 3 && (PL == PL_BGR8 || PL == PL_ANY)) {
+            // full_ok implies the 24-bit BGR layout (pair_fast) and 8-bit
+            // samples with no shift: a channel is
             // (int)clamp(x, 0, 255), i.e. the clamped truncation, which
             // v_cvt_pk_u8_f32 of trunc(x) computes and packs in one step
             float r0, g0, b0, r1, g1, b1;
