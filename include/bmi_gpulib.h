@@ -11,7 +11,9 @@
  * printed and called exit(-1) on any CUDA error (gpufunc.cu:9-22) or
  * asserted on an unsupported mode, these functions never exit: int
  * functions return a negative BMI_E* code, void functions record it, and
- * BMI_last_error() returns the most recent code of the calling thread.
+ * BMI_last_error() returns the most recent code of the calling thread
+ * (each void entry point of Part 1 resets it to BMI_OK on entry, so after
+ * such a call it tells whether that call failed).
  *
  * Part 2 is additive (no counterpart in the reference): the fused /
  * batched persistent decode, the plain-path driver used as its baseline,
@@ -173,6 +175,8 @@ typedef struct BmiTuning {
     int min_waves_per_cu; /* segment heights are chosen among those giving
                              each level step this many items per CU, or
                              the smallest height (default 4) */
+    int fixed_seg;       /* 1: every level step uses segments of exactly
+                            seg_rows output rows (the cost model is off) */
 } BmiTuning;
 
 typedef struct BmiBatchPlan BmiBatchPlan;

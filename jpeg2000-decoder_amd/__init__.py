@@ -93,6 +93,7 @@ class BmiTuning(ctypes.Structure):
         ("groups_per_cu", ctypes.c_int),
         ("persistent", ctypes.c_int),
         ("min_waves_per_cu", ctypes.c_int),
+        ("fixed_seg", ctypes.c_int),
     ]
 
 
@@ -192,6 +193,88 @@ class GpuInfo(ctypes.Structure):
     ]
 
 
+# ---------------------------------------------------------------------------
+# Types of the drop-in boundary (include/bmi_types.h; the reference's
+# decoder/cudainterface/include/cudainterfacedatatype.h:13-152)
+
+# CudaBufType
+HOST_BUF, HOST_ALLLIGN_BUF, GPU_BUF, GPU_ALLIGN_BUF = 0, 1, 2, 3
+# CudaCopyDirection
+COPY_H2D, COPY_D2H, COPY_H2H, COPY_D2D = 0, 1, 2, 3
+
+
+class S2d(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_short), ("y", ctypes.c_short)]
+
+
+class CudaBuf(ctypes.Structure):
+    _fields_ = [
+        ("mType", ctypes.c_int),
+        ("mSize", ctypes.c_uint),
+        ("mPitch", ctypes.c_uint),
+        ("mWidth", ctypes.c_uint),
+        ("mHeight", ctypes.c_uint),
+        ("mStatus", ctypes.c_int),
+        ("mAsyncId", ctypes.c_void_p),
+        ("mOwnerThreadIndex", ctypes.c_int),
+        ("mBufContain", ctypes.c_uint),
+        ("mBuf", ctypes.c_void_p),
+    ]
+
+
+class _StepUnion(ctypes.Union):
+    _fields_ = [("miAbsStep", ctypes.c_int), ("mfAbsStepNorm", ctypes.c_float)]
+
+
+class SubbandInfo_c(ctypes.Structure):
+    _anonymous_ = ("_u",)
+    _fields_ = [
+        ("mId", ctypes.c_int),
+        ("mOff", S2d),
+        ("mSize", S2d),
+        ("_u", _StepUnion),
+        ("mQquanShift", ctypes.c_int),
+        ("mLinesInserted", ctypes.c_int),
+    ]
+
+
+class _CompBufUnion(ctypes.Union):
+    _fields_ = [("mCompBuf", ctypes.POINTER(CudaBuf)), ("mDwtBuf", ctypes.c_void_p)]
+
+
+class CompInfo_c(ctypes.Structure):
+    _anonymous_ = ("_u",)
+    _fields_ = [
+        ("mSubbandInfo", ctypes.POINTER(SubbandInfo_c)),
+        ("mNumOfSubband", ctypes.c_int),
+        ("mSubbandFinishedBitmap", ctypes.c_int),
+        ("mResolutionFinishedId", ctypes.c_int),
+        ("mSubbandFinishedCounter", ctypes.c_int),
+        ("mAsyncOperationStreamIndex", ctypes.c_int),
+        ("_u", _CompBufUnion),
+        ("mDeShifted", ctypes.c_ubyte),
+        ("mCompBufMoved", ctypes.c_ubyte),
+    ]
+
+
+class _ResultBufUnion(ctypes.Union):
+    _fields_ = [("mResultBuf", ctypes.POINTER(CudaBuf)), ("mCpuResultBuf", ctypes.c_void_p)]
+
+
+class TileInfo_c(ctypes.Structure):
+    _anonymous_ = ("_u",)
+    _fields_ = [
+        ("mOff", S2d),
+        ("mSize", S2d),
+        ("mCompInfo", ctypes.POINTER(CompInfo_c)),
+        ("mComFinishedCounter", ctypes.c_int),
+        ("mLastCompId", ctypes.c_int),
+        ("_u", _ResultBufUnion),
+        ("mXParity", ctypes.c_ubyte * MAX_DWT_LEVEL),
+        ("mYParity", ctypes.c_ubyte * MAX_DWT_LEVEL),
+    ]
+
+
 _LIB = None
 
 
@@ -224,6 +307,39 @@ def lib() -> ctypes.CDLL:
     L.GpuInit.restype = i
     L.GpuRemove.argtypes = [ctypes.POINTER(GpuInfo)]
     L.GpuRemove.restype = i
+    bp, tp, cp = ctypes.POINTER(CudaBuf), ctypes.POINTER(TileInfo_c), ctypes.POINTER(CompInfo_c)
+    sp = ctypes.POINTER(vp)
+    sh = ctypes.c_short
+    L.MallocGpuBuf.argtypes = [bp]
+    L.MallocGpuBuf.restype = None
+    L.FreeGpuBuf.argtypes = [bp]
+    L.FreeGpuBuf.restype = None
+    L.CreateANewStream.argtypes = []
+    L.CreateANewStream.restype = vp
+    L.DestroyStream.argtypes = [sp]
+    L.DestroyStream.restype = None
+    L.GpuSync.argtypes = [sp]
+    L.GpuSync.restype = None
+    L.Gpu2DCopy.argtypes = [bp, bp, i, i, i, i, i, i, sp, i, i, i]
+    L.Gpu2DCopy.restype = None
+    L.Gpu2DCopyWithDiffPitch.argtypes = [bp, bp, i, i, i, i, i, i, i, i, sp, i, i, i]
+    L.Gpu2DCopyWithDiffPitch.restype = None
+    L.GpuLinearMemCopy.argtypes = [bp, bp, i, vp, i]
+    L.GpuLinearMemCopy.restype = None
+    L.BMI_comp_idwt.argtypes = [tp, cp, bp, bp, vp, sh, i, i]
+    L.BMI_comp_idwt.restype = i
+    L.BMI_resolution_idwt.argtypes = [tp, cp, bp, bp, vp, sh, i, i]
+    L.BMI_resolution_idwt.restype = i
+    L.BMI_tile_MCT.argtypes = [tp, sh, sh, i, i, i, vp]
+    L.BMI_tile_MCT.restype = None
+    L.BMI_tile_ICT.argtypes = [tp, sh, sh, i, i, i, vp]
+    L.BMI_tile_ICT.restype = None
+    L.BMI_tile_Merge.argtypes = [tp, i, i, i, sh, sh, i, i, i, vp]
+    L.BMI_tile_Merge.restype = None
+    L.BMI_tile_One_Component.argtypes = [tp, i, i, i, i, i, i, i, vp]
+    L.BMI_tile_One_Component.restype = None
+    L.BMI_Comp_DC_SHIFT.argtypes = [bp, bp, i, i, i, i, i, i, i, i, i, vp]
+    L.BMI_Comp_DC_SHIFT.restype = None
     L.BMI_batch_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(BmiTileDesc), i,
                                         ctypes.POINTER(BmiTuning)]
     L.BMI_batch_plan_create.restype = i

@@ -1068,7 +1068,7 @@ struct BmiBatchPlan {
     std::vector<Launch> launches;  // level-step mode: per step and variant
     std::vector<int> job_tile;                // tile of each job
     struct {
-        int seg_rows, waves_per_group, groups_per_cu, persistent, min_waves_per_cu;
+        int seg_rows, waves_per_group, groups_per_cu, persistent, min_waves_per_cu, fixed_seg;
     } tuning;
     LevelJob *d_jobs = nullptr;
     WorkItem *d_items = nullptr;
@@ -1192,6 +1192,7 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
     P->tuning.groups_per_cu = (tuning && tuning->groups_per_cu > 0) ? tuning->groups_per_cu : 8;
     P->tuning.persistent = tuning ? tuning->persistent : 0;
     P->tuning.min_waves_per_cu = (tuning && tuning->min_waves_per_cu > 0) ? tuning->min_waves_per_cu : 4;
+    P->tuning.fixed_seg = (tuning && tuning->fixed_seg) ? 1 : 0;
     if (P->tuning.waves_per_group > 4) P->tuning.waves_per_group = 4;
     if (hipGetDevice(&P->device) != hipSuccess) P->device = 0;
     {
@@ -1462,7 +1463,7 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                     max_pairs = std::max(max_pairs, (J.b1 - (J.b0 & ~1) + 1) / 2);
                 }
             }
-            if (any) {
+            if (any && !P->tuning.fixed_seg) {
                 const int halo = any97 ? 8 : 4;
                 auto items_at = [&](int sp) {
                     long total = fixed;

@@ -303,6 +303,18 @@ using namespace bmi;
 
 static int level_count(const CompInfo_c *comp) { return (comp->mNumOfSubband - 1) / 3; }
 
+// True when the region of `h` rows of `wbytes` bytes starting at byte x0 of
+// row y0 lies inside `b` addressed with `pitch` bytes per row.  Buffers
+// whose mSize is 0 (described by the caller, not allocated by
+// MallocGpuBuf) are not checked.
+static bool region_fits(const CudaBuf *b, size_t pitch, long long x0, long long y0,
+                        long long wbytes, long long h)
+{
+    if (!b->mSize || wbytes <= 0 || h <= 0) return true;
+    if (x0 < 0 || y0 < 0) return false;
+    return (unsigned long long)(y0 + h - 1) * pitch + x0 + wbytes <= b->mSize;
+}
+
 // One synthesis level `level` of a tile-component, in place in orig.
 static int plain_level(TileInfo_c *tile, CompInfo_c *comp, CudaBuf *orig, CudaBuf *temp,
                        hipStream_t s, EncodeMathod method, int level, int coef_kind, int r53_mode)
@@ -316,6 +328,9 @@ static int plain_level(TileInfo_c *tile, CompInfo_c *comp, CudaBuf *orig, CudaBu
     const int px = tile->mXParity[level] & 1, py = tile->mYParity[level] & 1;
     const int opitch = (int)(orig->mPitch / 4), tpitch = (int)(temp->mPitch / 4);
     if (tpitch < nh || opitch < nw) return bmi_fail(BMI_E_INVALID, "comp_idwt: pitch too small");
+    if (!region_fits(orig, orig->mPitch, 4LL * tile->mOff.x, tile->mOff.y, 4LL * nw, nh) ||
+        !region_fits(temp, temp->mPitch, 4LL * tile->mOff.y, tile->mOff.x, 4LL * nh, nw))
+        return bmi_fail(BMI_E_INVALID, "comp_idwt: tile outside its buffers");
     char *obase = (char *)orig->mBuf + (size_t)orig->mPitch * tile->mOff.y + 4 * (size_t)tile->mOff.x;
     char *tbase = (char *)temp->mBuf + (size_t)temp->mPitch * tile->mOff.x + 4 * (size_t)tile->mOff.y;
 
@@ -480,6 +495,11 @@ static void tile_rgb(TileInfo_c *tile, short bitDepth, short wordShift, int dwtL
     A.in_pitch = (int)(tile->mCompInfo[0].mCompBuf->mPitch / 4);
     A.w = w;
     A.h = h;
+    if (!region_fits(tile->mResultBuf, tile->mResultBuf->mPitch, (long long)rx * pb, ry,
+                     (long long)w * pb, h)) {
+        bmi_fail(BMI_E_INVALID, "tile colour: result region outside the result buffer");
+        return;
+    }
     A.out_pitch_px = (int)(tile->mResultBuf->mPitch / pb);
     A.out = (uint8_t *)tile->mResultBuf->mBuf + ((size_t)ry * A.out_pitch_px + rx) * pb;
     A.fmt = fmt;
@@ -497,12 +517,14 @@ static void tile_rgb(TileInfo_c *tile, short bitDepth, short wordShift, int dwtL
 extern "C" void BMI_tile_MCT(TileInfo_c *tileInfo, short bitDepth, short wordShift, int dwtlevel,
                              int decodeLevel, OutputFormat outFormat, StreamId asyncId)
 {
+    bmi_clear_error();
     tile_rgb(tileInfo, bitDepth, wordShift, dwtlevel, decodeLevel, outFormat, asyncId, false, -1);
 }
 
 extern "C" void BMI_tile_ICT(TileInfo_c *tileInfo, short bitDepth, short wordShift, int dwtlevel,
                              int decodeLevel, OutputFormat outFormat, StreamId asyncId)
 {
+    bmi_clear_error();
     tile_rgb(tileInfo, bitDepth, wordShift, dwtlevel, decodeLevel, outFormat, asyncId, true, -1);
 }
 
@@ -546,6 +568,11 @@ static void tile_merge(TileInfo_c *tile, int componentNum, int dwtLevel, int dec
     A.in_pitch = pitch / 4;
     A.w = w;
     A.h = h;
+    if (!region_fits(tile->mResultBuf, tile->mResultBuf->mPitch, (long long)rx * pb, ry,
+                     (long long)w * pb, h)) {
+        bmi_fail(BMI_E_INVALID, "tile merge: result region outside the result buffer");
+        return;
+    }
     A.out_pitch_px = (int)(tile->mResultBuf->mPitch / pb);
     A.out = (uint8_t *)tile->mResultBuf->mBuf + ((size_t)ry * A.out_pitch_px + rx) * pb;
     A.fmt = fmt;
@@ -563,6 +590,7 @@ extern "C" void BMI_tile_Merge(TileInfo_c *tileInfo, int componentNum, int dwtLe
                                int compBufPitchBytes, OutputFormat outFormat, EncodeMathod method,
                                StreamId asyncId)
 {
+    bmi_clear_error();
     tile_merge(tileInfo, componentNum, dwtLevel, decodeLevel, bitDepth, wordShift,
                compBufPitchBytes, outFormat, method, asyncId, -1);
 }
@@ -572,6 +600,7 @@ extern "C" void BMI_Comp_DC_SHIFT(CudaBuf *compBuf, CudaBuf *compBufTemp, Encode
                                   int width, int height, int bitDepth, int pixelWordLength,
                                   StreamId asyncId)
 {
+    bmi_clear_error();
     if (!compBuf || !compBufTemp || !compBuf->mBuf || !compBufTemp->mBuf || pixelWordLength != 4 ||
         width < 0 || height < 0 || bitDepth < 1 || bitDepth > 30) {
         bmi_fail(BMI_E_INVALID, "Comp_DC_SHIFT: bad args");
@@ -579,6 +608,11 @@ extern "C" void BMI_Comp_DC_SHIFT(CudaBuf *compBuf, CudaBuf *compBufTemp, Encode
     }
     if (!width || !height) return;
     // both buffers addressed with compBuf's pitch, as gpufunc.cu:397-413
+    if (!region_fits(compBuf, compBuf->mPitch, 4LL * src_x_off, src_y_off, 4LL * width, height) ||
+        !region_fits(compBufTemp, compBuf->mPitch, 4LL * dst_x_off, dst_y_off, 4LL * width, height)) {
+        bmi_fail(BMI_E_INVALID, "Comp_DC_SHIFT: region outside a buffer");
+        return;
+    }
     hipLaunchKernelGGL(plain_dc_shift_kernel, dim3((width + 255) / 256, height), dim3(256), 0,
                        bmi_stream(asyncId), compBuf->mBuf, (int *)compBufTemp->mBuf, src_x_off,
                        src_y_off, dst_x_off, dst_y_off, (int)(compBuf->mPitch / pixelWordLength),
@@ -590,6 +624,7 @@ extern "C" void BMI_tile_One_Component(TileInfo_c *tile, int component_id, Encod
                                        int hasMCT, int bitDepth, int wordShift, int dwtLevel,
                                        int decodeLevel, StreamId asyncId)
 {
+    bmi_clear_error();
     (void)dwtLevel;
     if (!tile || !tile->mCompInfo || !tile->mResultBuf || !tile->mResultBuf->mBuf ||
         component_id < 0 || component_id > 3 || wordShift != 2) {
@@ -626,6 +661,14 @@ extern "C" void BMI_tile_One_Component(TileInfo_c *tile, int component_id, Encod
         mode = method == Ckernels_W5X3 ? 1 : 2;
         in = (const int *)b->mBuf;
         pitch = (int)(b->mPitch / 4);
+    }
+    // the kernels index the result with the same pitch as their input
+    // (gpufunc.cu:1443-1455): the whole region must fit the result buffer
+    if (!region_fits(tile->mResultBuf, 4LL * pitch, 4LL * tile->mOff.x, tile->mOff.y, 4LL * w, h) ||
+        (in && !region_fits(tile->mCompInfo[component_id].mCompBuf, 4LL * pitch,
+                            4LL * tile->mOff.x, tile->mOff.y, 4LL * w, h))) {
+        bmi_fail(BMI_E_INVALID, "tile_One_Component: region outside a buffer");
+        return;
     }
     hipLaunchKernelGGL(plain_one_comp_kernel, dim3((w + 255) / 256, h), dim3(256), 0,
                        bmi_stream(asyncId), mode, component_id, in, (int *)tile->mResultBuf->mBuf,

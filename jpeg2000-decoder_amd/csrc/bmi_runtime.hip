@@ -108,6 +108,7 @@ extern "C" int GpuRemove(GpuInfo *gpu)
 // gpufunc.cu:134-187
 extern "C" void MallocGpuBuf(CudaBuf *buf)
 {
+    bmi_clear_error();
     if (!buf || buf->mBuf) {
         bmi_fail(BMI_E_INVALID, "MallocGpuBuf: null or already allocated");
         return;
@@ -144,6 +145,7 @@ extern "C" void MallocGpuBuf(CudaBuf *buf)
 
 extern "C" void FreeGpuBuf(CudaBuf *buf)
 {
+    bmi_clear_error();
     if (!buf || !buf->mBuf) return;
     hipError_t e = hipSuccess;
     switch (buf->mType) {
@@ -172,11 +174,13 @@ extern "C" StreamId CreateANewStream(void)
 
 extern "C" void DestroyStream(StreamId *s)
 {
+    bmi_clear_error();
     if (s && *s) bmi_check_hip(hipStreamDestroy((hipStream_t)*s), "DestroyStream");
 }
 
 extern "C" void GpuSync(StreamId *s)
 {
+    bmi_clear_error();
     if (s)
         bmi_check_hip(hipStreamSynchronize((hipStream_t)*s), "GpuSync(stream)");
     else
@@ -219,6 +223,7 @@ extern "C" void Gpu2DCopy(CudaBuf *srcBuf, CudaBuf *destBuf, int src_x_off, int 
                           StreamId *P_asyncId, CudaCopyDirection direction, int src_no_pitch,
                           int dest_no_pitch)
 {
+    bmi_clear_error();
     if (!srcBuf || !destBuf || width < 0 || height < 0) {
         bmi_fail(BMI_E_INVALID, "Gpu2DCopy: bad args");
         return;
@@ -237,6 +242,7 @@ extern "C" void Gpu2DCopyWithDiffPitch(CudaBuf *srcBuf, CudaBuf *destBuf, int sr
                                        CudaCopyDirection direction, int src_no_pitch,
                                        int dest_no_pitch)
 {
+    bmi_clear_error();
     if (!srcBuf || !destBuf || width < 0 || height < 0) {
         bmi_fail(BMI_E_INVALID, "Gpu2DCopyWithDiffPitch: bad args");
         return;
@@ -252,6 +258,7 @@ extern "C" void Gpu2DCopyWithDiffPitch(CudaBuf *srcBuf, CudaBuf *destBuf, int sr
 extern "C" void GpuLinearMemCopy(CudaBuf *srcBuf, CudaBuf *destBuf, int copy_size_in_byte,
                                  StreamId async_id, CudaCopyDirection direction)
 {
+    bmi_clear_error();
     if (!srcBuf || !destBuf || copy_size_in_byte < 0) {
         bmi_fail(BMI_E_INVALID, "GpuLinearMemCopy: bad args");
         return;

@@ -21,6 +21,11 @@
  *  - RCT: exhaustive forward/inverse identity on 8-bit and sampled 12-bit.
  *  - ICT / merge / DC shift: FP64 formula, exhaustive grid over the clamp
  *    and truncation order.
+ *  - One-component grey preview: byte extraction against numpy bit
+ *    operations on packed words; the 9/7 fixed-point reading against
+ *    round-half-away-from-zero of w / 2^13 on a dense sample and at the
+ *    half-way points; the 5/3 shift against the 8-bit slot rule at 4..16
+ *    bits (tests/test_oracle.py).
  */
 #include "j2k_oracle.h"
 
@@ -709,5 +714,48 @@ void or_dc_shift(int32_t *dst, int dst_pitch, const uint32_t *src, int src_pitch
             t += 1 << (bd - 1);
             t = t < 0 ? 0 : (t > mx ? mx : t);
             dst[(size_t)y * dst_pitch + x] = t;
+        }
+}
+
+void or_one_component(int32_t *out, int out_pitch, const int32_t *in, int in_pitch, int w, int h,
+                      int mode, int comp_id, int bd)
+{
+    /* BMI_tile_One_Component (gpufunc.cu:1404-1480), kernels
+       bmi_tile_one_component_int / _float and bmi_get_one_component_from_result
+       (bmi_dwt_gpu_kernel.h:1644-1836).  Every mode writes the grey word
+       o | o<<8 | o<<16.
+       mode 0: o = byte (2 - comp_id) of the packed 32-bit result word `in`.
+       mode 1 (5/3 plane): comp 0 adds 2^(B-1), other comps add 2^B - 1 and
+         halve; clamp to [0, 2^B-1]; to 8 bits (>> B-8, or << 8-B); & 0xff.
+       mode 2 (9/7 kernel): the word is read as a fixed-point value with 13
+         fraction bits, rounded half away from zero to an integer, + 2^(B-1),
+         clamped to [0, 2^B-1] and shifted right by B-8 (B >= 8). */
+    const int mx = (1 << bd) - 1;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const int32_t v = in[(size_t)y * in_pitch + x];
+            int o;
+            if (mode == 0) {
+                o = (int)(((uint32_t)v >> ((2 - comp_id) * 8)) & 0xffu);
+            } else if (mode == 1) {
+                int t = v;
+                if (comp_id == 0) t += 1 << (bd - 1);
+                else t = (t + (1 << bd) - 1) >> 1;
+                t = t < 0 ? 0 : (t > mx ? mx : t);
+                t = bd >= 8 ? (t >> (bd - 8)) : (t << (8 - bd));
+                o = t & 0xff;
+            } else {
+                /* flag * ((flag * w + 0x1000) & 0xfffe000) >> 13, with the
+                   products in 32-bit two's complement */
+                const int neg = v < 0;
+                uint32_t m = neg ? (uint32_t)0 - (uint32_t)v : (uint32_t)v;
+                m = (m + 0x1000u) & 0xfffe000u;
+                int32_t t = neg ? -(int32_t)m : (int32_t)m;
+                t >>= 13;
+                t += 1 << (bd - 1);
+                t = t < 0 ? 0 : (t > mx ? mx : t);
+                o = t >> (bd - 8);
+            }
+            out[(size_t)y * out_pitch + x] = o | (o << 8) | (o << 16);
         }
 }

@@ -86,6 +86,11 @@ int or_merge_pack(uint8_t *out, int out_pitch_px, const void *const *comps, int 
 /* bmi_dc_shift_kernel semantics (decoder/cudainterface/src/bmi_dwt_gpu_kernel.h:709-751) */
 void or_dc_shift(int32_t *dst, int dst_pitch, const uint32_t *src, int src_pitch, int w, int h,
                  int bit_depth, int is_float);
+/* grey preview of one component (BMI_tile_One_Component, gpufunc.cu:1404-1480,
+   bmi_dwt_gpu_kernel.h:1644-1836); mode 0 from packed 32-bit result words,
+   1 from a 5/3 integer plane, 2 the 9/7 kernel's fixed-point reading */
+void or_one_component(int32_t *out, int out_pitch, const int32_t *in, int in_pitch, int w, int h,
+                      int mode, int comp_id, int bit_depth);
 /* the unclamped ICT value before truncation, for pre-clamp parity checks */
 void or_ict_float(float *r, float *g, float *b, const float *c0, const float *c1,
                   const float *c2, int n, int bit_depth);

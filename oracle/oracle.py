@@ -50,6 +50,7 @@ def lib() -> ctypes.CDLL:
         L.or_ict_pack.argtypes = [P, i, P, P, P, i, i, i, i, i, i]
         L.or_merge_pack.argtypes = [P, i, P, i, i, i, i, i, i, i, i]
         L.or_dc_shift.argtypes = [P, i, P, i, i, i, i, i]
+        L.or_one_component.argtypes = [P, i, P, i, i, i, i, i, i]
         L.or_ict_float.argtypes = [P, P, P, P, P, P, i, i]
         L.or_pixel_bytes.argtypes = [i]
         L.or_t1_decode_block.argtypes = [P, ctypes.c_long, i, i, i, i, i, i, P, i]
@@ -182,6 +183,16 @@ def dc_shift(plane: np.ndarray, bit_depth: int, is_float: bool):
     h, w = a.shape
     out = np.zeros((h, w), np.int32)
     lib().or_dc_shift(_p(out), w, _p(a), w, w, h, bit_depth, int(is_float))
+    return out
+
+
+def one_component(words: np.ndarray, mode: int, comp_id: int, bit_depth: int) -> np.ndarray:
+    """Grey preview words of BMI_tile_One_Component: mode 0 from packed
+    32-bit result words, 1 from a 5/3 plane, 2 the 9/7 fixed-point kernel."""
+    a = np.ascontiguousarray(words).view(np.int32)
+    h, w = a.shape
+    out = np.zeros((h, w), np.int32)
+    lib().or_one_component(_p(out), w, _p(a), w, w, h, mode, comp_id, bit_depth)
     return out
 
 

@@ -48,7 +48,7 @@ def test_struct_layouts(pkg):
     assert d.coef_pitch.offset == 32
     assert d.steps.offset == 92
     assert d.out.offset == 400
-    assert ctypes.sizeof(pkg.BmiTuning) == 20
+    assert ctypes.sizeof(pkg.BmiTuning) == 24
 
 
 def test_error_strings_without_gpu(pkg):
@@ -88,3 +88,56 @@ def test_t1_order_blocks_longest_first(pkg):
     assert got == sorted(((n, i) for i, n in enumerate(lens)), key=lambda x: -x[0])
     assert pkg.lib().BMI_t1_order_blocks(None, 0) == 0
     assert pkg.lib().BMI_t1_order_blocks(None, 3) != 0
+
+
+def _c_layout(tmp_path, structs):
+    """sizeof / offsetof of the named fields as the C compiler lays them out
+    from include/bmi_types.h and include/bmi_gpulib.h."""
+    import subprocess
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "bmi_gpulib.h"',
+             'int main(void) {']
+    for sname, fields in structs.items():
+        lines.append(f'printf("{sname} size %zu\\n", sizeof({sname}));')
+        for f in fields:
+            lines.append(f'printf("{sname} {f} %zu\\n", offsetof({sname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", str(ROOT / "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    res = {}
+    for ln in out.splitlines():
+        s, f, v = ln.split()
+        res[(s, f)] = int(v)
+    return res
+
+
+def test_ctypes_mirrors_match_c_layout(pkg, tmp_path):
+    """Every field of the ctypes mirrors of the ABI structs (the reference's
+    cudainterfacedatatype.h types and the additive descriptors) sits at the
+    offset the C compiler gives it."""
+    mirrors = {
+        "CudaBuf": pkg.CudaBuf, "SubbandInfo_c": pkg.SubbandInfo_c,
+        "CompInfo_c": pkg.CompInfo_c, "TileInfo_c": pkg.TileInfo_c, "GpuInfo": pkg.GpuInfo,
+        "BmiTileDesc": pkg.BmiTileDesc, "BmiTuning": pkg.BmiTuning,
+        "BmiServerConfig": pkg.BmiServerConfig,
+    }
+    anon = {"miAbsStep": "mfAbsStepNorm"}
+
+    def fields(cls):
+        out = []
+        for name, _t in cls._fields_:
+            if name == "_u":
+                out += [n for n, _ in _t._fields_]
+            else:
+                out.append(name)
+        return out
+
+    structs = {s: [f for f in fields(c) if f not in anon] for s, c in mirrors.items()}
+    lay = _c_layout(tmp_path, structs)
+    for s, cls in mirrors.items():
+        assert ctypes.sizeof(cls) == lay[(s, "size")], s
+        for f in structs[s]:
+            assert getattr(cls, f).offset == lay[(s, f)], (s, f)

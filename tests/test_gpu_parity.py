@@ -135,15 +135,35 @@ def test_signmag_steps_match_oracle(pkg, workloads, orc):
         _check_planes(got, exp, "97", run.__name__)
 
 
+def _items(pkg, planes, tuning, **kw):
+    from gpu_helpers import make_case
+    desc, _keep = make_case(pkg, planes, **kw)
+    plan = pkg.BatchPlan([desc], tuning)
+    n = plan.num_items
+    plan.close()
+    return n
+
+
 @pytest.mark.parametrize("seg_rows", [2, 6, 16, 64, 256])
 @pytest.mark.parametrize("persistent", [0, 1])
 def test_fused_launch_shapes_agree(pkg, workloads, orc, seg_rows, persistent):
+    """Fixed segment heights (fixed_seg = 1 turns the cost model off, so
+    each height reaches the kernel: the item count shows it) give the
+    oracle's pixels, for step and persistent launches."""
     w, h, L = 301, 199, 5
     planes = workloads.tile_planes(w, h, L, "97", 3, seed=8, ll97=(0.0, 40.0))
     _, exp = orc.decode_tile(planes, levels=L, filt="97", out_format=2)
-    t = pkg.BmiTuning(seg_rows, 4, 4, persistent, 0)
+    t = pkg.BmiTuning(seg_rows, 4, 4, persistent, 0, 1)
     _, got = run_fused(pkg, planes, tuning=t, filt="97", levels=L, out_format=2)
     assert np.array_equal(got, exp), diff_report(got, exp)
+    # final level alone: ceil(199 / seg_rows) segments of each of 3 strips
+    # (ceil(301 / 120)); the four coarser levels add theirs
+    n = _items(pkg, planes, t, filt="97", levels=L, out_format=2)
+    final = 3 * (-(-h // seg_rows))
+    assert n >= final, (n, final)
+    if seg_rows == 2:
+        assert n > _items(pkg, planes, pkg.BmiTuning(64, 4, 4, persistent, 0, 1), filt="97",
+                          levels=L, out_format=2)
 
 
 @pytest.mark.parametrize("w", [300, 301])
@@ -170,7 +190,40 @@ def test_alternating_segment_directions_match_oracle(pkg, workloads, orc, seg_ro
     planes = workloads.tile_planes(w, h, L, "97", 2, seed=seg_rows * 10 + y0, y0=y0)
     exp, _ = orc.decode_tile(planes, levels=L, filt="97", out_format=7, y0=y0)
     for persistent in (0, 1):
-        t = pkg.BmiTuning(seg_rows, 0, 0, persistent, 1)
+        t = pkg.BmiTuning(seg_rows, 0, 0, persistent, 1, 1)
         got, _ = run_fused(pkg, planes, tuning=t, filt="97", levels=L, out_format=pkg.FMT_NONE,
                            y0=y0)
         _check_planes(got, exp, "97", f"seg {seg_rows} y0 {y0} persistent {persistent}")
+
+
+@pytest.mark.parametrize("mode", [2, 0])
+def test_ict_saturation_in_mixed_batch(pkg, workloads, orc, mode):
+    """The saturating 9/7 + ICT case in one plan beside a 5/3 RCT tile and a
+    grey 9/7 tile: mode 2 runs every step as one dispatching launch (its
+    packed 24-bit 9/7 path clamps at 0 and 255), mode 0 lets the plan choose.
+    Every tile equals the oracle bit for bit."""
+    import torch
+    from gpu_helpers import fetch, make_case
+    specs = [
+        dict(w=301, h=200, L=3, filt="97", nc=3, fmt=2, co=-1,
+             kw=dict(ll97=(-300.0, 500.0))),
+        dict(w=150, h=97, L=4, filt="53", nc=3, fmt=2, co=-1, kw=dict(ll53=(-200, 200))),
+        dict(w=77, h=130, L=3, filt="97", nc=1, fmt=6, co=0, kw=dict(ll97=(-300.0, 500.0))),
+    ]
+    cases, exps = [], []
+    for i, s in enumerate(specs):
+        planes = workloads.tile_planes(s["w"], s["h"], s["L"], s["filt"], s["nc"], seed=90 + i,
+                                       **s["kw"])
+        exps.append(orc.decode_tile(planes, levels=s["L"], filt=s["filt"], out_format=s["fmt"],
+                                    check_out=s["co"])[1])
+        cases.append(make_case(pkg, planes, filt=s["filt"], levels=s["L"], out_format=s["fmt"],
+                               check_out=s["co"]))
+    assert (exps[0] == 0).mean() > 0.3 and (exps[0] == 255).mean() > 0.1
+    plan = pkg.BatchPlan([d for d, _ in cases], pkg.BmiTuning(32, 4, 4, mode, 0, 0))
+    plan.execute(0)
+    torch.cuda.synchronize()
+    assert plan.error_flag() == 0
+    plan.close()
+    for i, ((_, keep), exp) in enumerate(zip(cases, exps)):
+        _, got = fetch(keep, specs[i]["filt"])
+        assert np.array_equal(got, exp), f"tile {i}: {diff_report(got, exp)}"

@@ -287,6 +287,55 @@ def test_dc_shift(orc):
     assert orc.dc_shift(f, 8, True).tolist() == [[128, 129, 0]]
 
 
+def test_one_component_from_result_extracts_the_byte(orc):
+    """Mode 0 reads byte (2 - comp) of each packed 0x00RRGGBB word: against
+    numpy bit operations on words built from known channels."""
+    rng = np.random.default_rng(4)
+    r, g, b = (rng.integers(0, 256, (7, 9)) for _ in range(3))
+    words = ((r << 16) | (g << 8) | b | (rng.integers(0, 256, (7, 9)) << 24)).astype(np.uint32)
+    for comp, ch in enumerate((r, g, b)):
+        got = orc.one_component(words, 0, comp, 8)
+        assert np.array_equal(got, ch | (ch << 8) | (ch << 16)), comp
+
+
+def test_one_component_97_fixed_point_rounding(orc):
+    """Mode 2 reads the word as fixed point with 13 fraction bits and rounds
+    half away from zero: against np.floor(|w| / 2^13 + 1/2) with the sign,
+    on a dense sample and at every half-way point, then level-shifted,
+    clamped and reduced to 8 bits."""
+    rng = np.random.default_rng(5)
+    w = np.concatenate([rng.integers(-(1 << 24), 1 << 24, 20000),
+                        np.arange(-40, 41) * 4096, np.arange(-40, 41) * 4096 - 1,
+                        np.arange(-40, 41) * 4096 + 1, [-(1 << 31), (1 << 31) - 1]]).astype(np.int64)
+    for bd in (8, 10, 12, 16):
+        got = orc.one_component(w.astype(np.int32).reshape(1, -1), 2, 0, bd)[0]
+        mag = np.floor(np.abs(w) / 8192.0 + 0.5)
+        # the kernel keeps 15 integer bits of the rounded magnitude (mask 0xfffe000)
+        mag = np.mod(mag, 1 << 15)
+        v = (np.sign(w) * mag).astype(np.int64) + (1 << (bd - 1))
+        o = np.clip(v, 0, (1 << bd) - 1) >> (bd - 8)
+        assert np.array_equal(got, o | (o << 8) | (o << 16)), bd
+
+
+def test_one_component_53_to_8_bits(orc):
+    """Mode 1: component 0 is level-shifted, the others take (v + 2^B - 1) >> 1;
+    then clamp and the 8-bit slot rule (>> B-8 above 8 bits, << 8-B below),
+    at 4..16 bits, against a numpy statement of the same rule over the
+    whole input range."""
+    for bd in (4, 6, 8, 12, 16):
+        v = np.arange(-(1 << bd) - 3, (1 << bd) + 4, dtype=np.int64)
+        for comp in (0, 1):
+            got = orc.one_component(v.astype(np.int32).reshape(1, -1), 1, comp, bd)[0]
+            t = v + (1 << (bd - 1)) if comp == 0 else (v + (1 << bd) - 1) >> 1
+            t = np.clip(t, 0, (1 << bd) - 1)
+            t = (t >> (bd - 8)) if bd >= 8 else ((t << (8 - bd)) & 0xFF)
+            assert np.array_equal(got, t | (t << 8) | (t << 16)), (bd, comp)
+        # an 8-bit component 0 at its mid-grey and extremes
+        if bd == 8:
+            got = orc.one_component(np.array([[-128, 0, 127]], np.int32), 1, 0, 8)[0]
+            assert got.tolist() == [0, 0x808080, 0xFFFFFF]
+
+
 def test_97_error_budget_vs_f64(orc, workloads):
     """The stated 9/7 error bound: the float path (which the GPU matches
     bit for bit) against the FP64 lifting oracle on the image workload's
