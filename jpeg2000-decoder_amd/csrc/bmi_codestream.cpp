@@ -44,7 +44,9 @@ struct ParseError {
 
 [[noreturn]] void fail(int code, const std::string &what) { throw ParseError{code, what}; }
 
-inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+// ceil(a / b) for b > 0 and any sign of a (band origins of T.800 B-15 can
+// be negative before the division)
+inline int ceil_div(long long a, long long b) { return (int)(a >= 0 ? (a + b - 1) / b : -((-a) / b)); }
 inline int floor_log2(unsigned v)
 {
     int n = -1;
@@ -372,15 +374,24 @@ void parse_marker_segments(BmiCodestream *S, const uint8_t *d, size_t n)
         switch (mk) {
         case MK_SIZ: {
             r.u16();
-            S->Xsiz = r.u32(); S->Ysiz = r.u32(); S->XOsiz = r.u32(); S->YOsiz = r.u32();
-            S->XTsiz = r.u32(); S->YTsiz = r.u32(); S->XTOsiz = r.u32(); S->YTOsiz = r.u32();
+            // the eight 32-bit fields are unsigned (T.800 A.5.1); values
+            // above INT_MAX are refused so every later sum fits in 64 bits
+            // and every coordinate in an int
+            uint32_t f[8];
+            for (uint32_t &v : f) v = r.u32();
+            for (uint32_t v : f)
+                if (v > 0x7FFFFFFFu) fail(BMI_E_UNSUPPORTED, "SIZ value above 2^31 - 1");
+            S->Xsiz = (int)f[0]; S->Ysiz = (int)f[1]; S->XOsiz = (int)f[2]; S->YOsiz = (int)f[3];
+            S->XTsiz = (int)f[4]; S->YTsiz = (int)f[5]; S->XTOsiz = (int)f[6]; S->YTOsiz = (int)f[7];
             S->Csiz = r.u16();
             if (S->Csiz < 1 || S->Csiz > MAX_COMPONENT)
                 fail(BMI_E_UNSUPPORTED, "1..4 components supported");
+            // T.800 A.5.1: XOsiz < Xsiz, 0 < XTsiz, XTOsiz <= XOsiz,
+            // XTsiz + XTOsiz > XOsiz (and the same vertically)
             if (S->Xsiz <= S->XOsiz || S->Ysiz <= S->YOsiz || S->XTsiz <= 0 || S->YTsiz <= 0 ||
                 S->XTOsiz > S->XOsiz || S->YTOsiz > S->YOsiz ||
-                S->XTOsiz + S->XTsiz <= S->XOsiz || S->YTOsiz + S->YTsiz <= S->YOsiz ||
-                S->Xsiz < 0 || S->Ysiz < 0 || S->XTsiz < 0 || S->YTsiz < 0)
+                (long long)S->XTOsiz + S->XTsiz <= S->XOsiz ||
+                (long long)S->YTOsiz + S->YTsiz <= S->YOsiz)
                 fail(BMI_E_INVALID, "inconsistent SIZ geometry");
             for (int c = 0; c < S->Csiz; ++c) {
                 unsigned s = r.u8(), dx = r.u8(), dy = r.u8();
@@ -438,6 +449,10 @@ void parse_marker_segments(BmiCodestream *S, const uint8_t *d, size_t n)
     int ntx = ceil_div((long long)S->Xsiz - S->XTOsiz, S->XTsiz);
     int nty = ceil_div((long long)S->Ysiz - S->YTOsiz, S->YTsiz);
     if ((long long)ntx * nty > 65535) fail(BMI_E_INVALID, "more than 65535 tiles");
+    // a tile is at most XTsiz wide and lies inside the image; the back end
+    // (like the reference's 16-bit tile sizes) takes tiles up to 32767
+    if (std::min(S->XTsiz, S->Xsiz - S->XOsiz) > 32767 || std::min(S->YTsiz, S->Ysiz - S->YOsiz) > 32767)
+        fail(BMI_E_UNSUPPORTED, "tiles larger than 32767 samples");
     S->tiles.resize((size_t)ntx * nty);
     S->info.ntiles_x = ntx;
     S->info.ntiles_y = nty;
@@ -554,8 +569,8 @@ void band_quant(const QuantStyle &q, int bid, int levels, int depth, int reversi
 void build_tile(BmiCodestream *S, int t, Tile &T)
 {
     int p = t % S->info.ntiles_x, q = t / S->info.ntiles_x;
-    T.tx0 = std::max(S->XTOsiz + p * S->XTsiz, S->XOsiz);
-    T.ty0 = std::max(S->YTOsiz + q * S->YTsiz, S->YOsiz);
+    T.tx0 = (int)std::max((long long)S->XTOsiz + (long long)p * S->XTsiz, (long long)S->XOsiz);
+    T.ty0 = (int)std::max((long long)S->YTOsiz + (long long)q * S->YTsiz, (long long)S->YOsiz);
     T.tx1 = std::min((long long)S->XTOsiz + (long long)(p + 1) * S->XTsiz, (long long)S->Xsiz);
     T.ty1 = std::min((long long)S->YTOsiz + (long long)(q + 1) * S->YTsiz, (long long)S->Ysiz);
     T.comps.resize(S->Csiz);
@@ -887,6 +902,13 @@ extern "C" int BMI_codestream_tile_blocks(const BmiCodestream *S, int t, void *c
         if (seg_capacity < T.seg_bytes + 16) return bmi_fail(BMI_E_INVALID, "segment buffer below seg_bytes + 16");
         for (int c = 0; c < S->Csiz; ++c)
             if (!planes[c]) return bmi_fail(BMI_E_INVALID, "plane %d missing", c);
+        // every block's rectangle lies in the tile-sized planes: the geometry
+        // guarantees it, and the check keeps a wrong table from ever becoming
+        // an out-of-bounds store of the decoder
+        for (const Block &b : T.blocks)
+            if (b.x < 0 || b.y < 0 || b.w < 1 || b.h < 1 || b.w > 64 || b.h > 64 ||
+                (long long)b.x + b.w > w || (long long)b.y + b.h > h || b.comp < 0 || b.comp >= S->Csiz)
+                return bmi_fail(BMI_E_INVALID, "code-block outside its tile");
         // layout: the segment-length tables of segmented blocks, then
         // every block's data (its segments in order), then 16 bytes of 0xFF
         uint8_t *dst = static_cast<uint8_t *>(segs);

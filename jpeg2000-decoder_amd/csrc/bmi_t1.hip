@@ -8,10 +8,12 @@
 // lanes do the parallel parts: clearing the block state, and writing the
 // decoded block (one lane per column, coalesced rows).
 //
-// Per wave, LDS holds the sample words (up to 64x64 x 4 B) and one 16-bit
-// state word per sample on a grid with a one-sample border.  The state word
-// carries the significance of the eight neighbours, so a context is formed
-// from a single LDS read:
+// The stripe being decoded lives in registers, one column per lane: four
+// sample words and four 16-bit state words.  Across stripes, the state
+// words are kept in LDS (8 KB per wave) and the sample words in the block's
+// own output rows in global memory, loaded one stripe ahead of use and
+// stored back after it.  The state word carries the significance of the
+// eight neighbours, so a context is formed from one lookup:
 //   bits 0..7   significance of N, S, W, E, NW, NE, SW, SE
 //   bit 8       significant        bit 9   negative
 //   bit 10      coded in this plane's significance pass

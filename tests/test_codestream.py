@@ -411,6 +411,33 @@ def test_product_rejects_malformed_and_unsupported(pkg):
     assert pkg.Codestream(jt.rewrite(data, extra_main=com)).info.total_blocks > 0
 
 
+def _set_siz(data: bytes, **fields) -> bytes:
+    """Rewrite 32-bit SIZ fields (T.800 A.5.1) of a codestream in place."""
+    names = ["Xsiz", "Ysiz", "XOsiz", "YOsiz", "XTsiz", "YTsiz", "XTOsiz", "YTOsiz"]
+    at = data.index(b"\xff\x51") + 6
+    b = bytearray(data)
+    for k, v in fields.items():
+        i = at + 4 * names.index(k)
+        b[i:i + 4] = struct.pack(">I", v & 0xFFFFFFFF)
+    return bytes(b)
+
+
+def test_siz_fields_are_unsigned_and_bounded(pkg):
+    """SIZ's 32-bit fields are unsigned: values above 2^31 - 1 (which an int
+    would read as negative origins), origins at or past the image edge, tile
+    grids of more than 65535 tiles and tiles over 32767 samples are refused."""
+    data = (J2K / "grey_37x53.j2k").read_bytes()
+    _expect_error(pkg, _set_siz(data, XOsiz=0xFFFFFFF0, XTOsiz=0xFFFFFFF0), pkg.BMI_E_UNSUPPORTED)
+    _expect_error(pkg, _set_siz(data, Xsiz=0x80000000), pkg.BMI_E_UNSUPPORTED)
+    _expect_error(pkg, _set_siz(data, XOsiz=53), pkg.BMI_E_INVALID)          # Xsiz <= XOsiz
+    _expect_error(pkg, _set_siz(data, XTOsiz=5, XOsiz=3), pkg.BMI_E_INVALID)  # XTOsiz > XOsiz
+    _expect_error(pkg, _set_siz(data, XTsiz=0), pkg.BMI_E_INVALID)
+    big = _set_siz(data, Xsiz=70000, Ysiz=70000, XTsiz=70000, YTsiz=70000)
+    _expect_error(pkg, big, pkg.BMI_E_UNSUPPORTED)                            # a 70000-wide tile
+    many = _set_siz(data, Xsiz=70000, Ysiz=70000, XTsiz=200, YTsiz=200)
+    _expect_error(pkg, many, pkg.BMI_E_INVALID)                               # 122500 tiles
+
+
 def test_tile_blocks_argument_checks(pkg):
     cs = pkg.Codestream((J2K / "grey_37x53.j2k").read_bytes())
     _, _, w, h, nb, sb = cs.tile_info(0)

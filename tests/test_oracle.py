@@ -174,6 +174,28 @@ def test_97_line_mirror_symmetry(orc, n):
     np.testing.assert_array_equal(a, b[::-1])
 
 
+@pytest.mark.parametrize("w", [2, 3, 4, 5])
+@pytest.mark.parametrize("h", [2, 3, 4, 5])
+@pytest.mark.parametrize("x0,y0", [(0, 0), (1, 1), (1, 0), (0, 1)])
+def test_97_small_resolutions_follow_t800(orc, w, h, x0, y0):
+    """Widths and heights 2..5 are where the reference's 9/7 synthesis is
+    undefined: HDWT_R97 writes nothing at odd parity
+    (bmi_cpu_dwt_decoder.cpp:2307-2310); VDWT_R97 at even parity, heights 3
+    and 5, reads a high row past the band and writes one row past the
+    resolution (the `<=` loops at :3113, :3146); at odd parity, heights
+    2..5, its step-4/6 loops update row 0 again from row -1 (:3237-3243,
+    :3265-3270).  The oracle (and the GPU, which equals it bit for bit)
+    produces the T.800 F.3.8.2 synthesis with symmetric extension there,
+    pinned here by the FP64 analysis (T.800 F.4.8.2) followed by the
+    oracle's float synthesis returning the image, at both parities, for one
+    level (every line 2..5 samples) and two (the coarse level at 1..3)."""
+    img = np.random.default_rng(w * 10 + h + 100 * x0 + 1000 * y0).normal(0, 60, (h, w))
+    for levels in (1, 2):
+        coef = orc.fdwt97_f64(img, levels, x0=x0, y0=y0)
+        back = orc.idwt97(coef.astype(np.float32), levels, x0=x0, y0=y0)
+        assert np.abs(back.astype(np.float64) - img).max() < 2e-3, (levels,)
+
+
 def test_signmag_equals_float_input(orc, workloads):
     """Dequantising T1 sign-magnitude words equals feeding the same
     integers as floats (FETCHPIXEL_INT, bmi_cpu_dwt_decoder.cpp:2287)."""
