@@ -226,6 +226,21 @@ int BMI_autotune(const BmiTileDesc *tiles, int ntiles, const BmiTuning *candidat
    width*height*4 bytes (transposed scratch). */
 int BMI_plain_decode(const BmiTileDesc *tile, void *temp, size_t temp_bytes, StreamId stream);
 
+/* Decode `tiles` once, as BMI_batch_plan_create + BMI_batch_plan_execute +
+   BMI_batch_plan_destroy would, without allocating device memory.  No
+   counterpart in the reference: for callers that cannot allocate (code
+   under hipGraph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  The plan's scratch planes, tables and counters live in one
+   file-scope device array of BMI_decode_tiles_noalloc_capacity() bytes
+   (64 MiB) per GPU, held until the process ends; a batch needing more
+   returns BMI_E_NOMEM before any device work.  Asynchronous on `stream`
+   after a host-side table build.  Every call on a GPU shares that GPU's
+   array: calls that overlap on one GPU (from other threads, or on other
+   streams while earlier work still runs) race. */
+int BMI_decode_tiles_noalloc(const BmiTileDesc *tiles, int ntiles, const BmiTuning *tuning,
+                             StreamId stream);
+size_t BMI_decode_tiles_noalloc_capacity(void);
+
 /* ------------------------------------------------------------------------
  * Request server (continuous batching over a fixed HBM budget)
  * --------------------------------------------------------------------- */

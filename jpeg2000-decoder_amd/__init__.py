@@ -380,6 +380,10 @@ def lib() -> ctypes.CDLL:
     L.BMI_t1_num_segments.argtypes = [i, i]
     L.BMI_t1_order_blocks.argtypes = [ctypes.POINTER(BmiT1Block), i]
     L.BMI_t1_check_blocks.restype = i
+    L.BMI_decode_tiles_noalloc.argtypes = [ctypes.POINTER(BmiTileDesc), i, ctypes.POINTER(BmiTuning), vp]
+    L.BMI_decode_tiles_noalloc.restype = i
+    L.BMI_decode_tiles_noalloc_capacity.argtypes = []
+    L.BMI_decode_tiles_noalloc_capacity.restype = ctypes.c_size_t
     L.BMI_plain_decode.argtypes = [ctypes.POINTER(BmiTileDesc), vp, ctypes.c_size_t, vp]
     L.BMI_plain_decode.restype = i
     L.BMI_codestream_open.argtypes = [vp, ctypes.c_size_t, i, ctypes.POINTER(vp),
@@ -503,6 +507,14 @@ def autotune(descs, candidates=None, reps: int = 5, stream: int = 0):
     check(lib().BMI_autotune(arr, len(descs), cand, n, reps, stream or None, ctypes.byref(best),
                              ctypes.byref(us)), "BMI_autotune")
     return best, float(us.value)
+
+
+def decode_tiles_noalloc(descs, tuning: BmiTuning | None = None, stream: int = 0) -> None:
+    """BMI_decode_tiles_noalloc: decode the tiles once with no device
+    allocation (the library's fixed per-GPU scratch array)."""
+    arr = (BmiTileDesc * len(descs))(*descs)
+    check(lib().BMI_decode_tiles_noalloc(arr, len(descs), ctypes.byref(tuning) if tuning else None,
+                                         stream or None), "BMI_decode_tiles_noalloc")
 
 
 def plain_decode(desc: BmiTileDesc, temp_ptr: int, temp_bytes: int, stream: int = 0) -> None:

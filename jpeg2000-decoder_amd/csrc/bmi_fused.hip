@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <type_traits>
@@ -1609,15 +1610,14 @@ size_t bmi_plan_estimate_bytes(const BmiTileDesc *t, const BmiTuning *tuning)
            align256(sizeof(int) * (jobs + 3));
 }
 
-int bmi_plan_bind(BmiBatchPlan *P, void *base, hipStream_t stream, void *staging,
-                  size_t staging_bytes)
+// Place scratch, job table, items and counters in the device_bytes at
+// `base` and relocate the scratch pointers of the job table (host only).
+static void plan_place(BmiBatchPlan *P, void *base)
 {
-    if (!P || !base || P->scratch) return bmi_fail(BMI_E_INVALID, "plan_bind: bad args");
     char *b = (char *)base;
     P->scratch = b;
     const size_t jb = sizeof(LevelJob) * P->jobs.size();
     const size_t ib = sizeof(WorkItem) * P->items.size();
-    const size_t cb = sizeof(int) * (P->jobs.size() + 3);
     size_t off = align256(P->scratch_bytes);
     P->d_jobs = (LevelJob *)(b + off);
     off += align256(jb);
@@ -1631,6 +1631,16 @@ int bmi_plan_bind(BmiBatchPlan *P, void *base, hipStream_t stream, void *staging
             for (int s = 0; s < nslots; ++s) J.ll[s] = (const char *)J.ll[s] + (uintptr_t)b;
         if (J.out_kind == OUT_SCRATCH) J.dst[0] = (char *)J.dst[0] + (uintptr_t)b;
     }
+}
+
+int bmi_plan_bind(BmiBatchPlan *P, void *base, hipStream_t stream, void *staging,
+                  size_t staging_bytes)
+{
+    if (!P || !base || P->scratch) return bmi_fail(BMI_E_INVALID, "plan_bind: bad args");
+    plan_place(P, base);
+    const size_t jb = sizeof(LevelJob) * P->jobs.size();
+    const size_t ib = sizeof(WorkItem) * P->items.size();
+    const size_t cb = sizeof(int) * (P->jobs.size() + 3);
     if (staging && staging_bytes >= align256(jb) + ib) {
         // one copy of both tables from pinned memory (same layout as on the device)
         memcpy(staging, P->jobs.data(), jb);
@@ -1948,4 +1958,108 @@ extern "C" int BMI_autotune(const BmiTileDesc *tiles, int ntiles, const BmiTunin
     *best = cand[best_i];
     if (best_us) *best_us = best_t;
     return BMI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Decode without allocating: the plan's device memory (ping-pong scratch
+// planes, job and item tables, persistent-mode counters) lives in one
+// file-scope device array instead of a hipMalloc'd block.  For callers that
+// cannot call hipMalloc (code under graph capture, tight serving loops,
+// harnesses that forbid it).  The array exists once per GPU, from module
+// load to process end; every call on a GPU reuses it, so calls that overlap
+// on one GPU (other host threads, or other streams whose work is still
+// running) race.  Host-side tables are staged in a per-device host buffer
+// that the next call reuses only after the previous upload has been read.
+
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+__global__ void zero_ints_kernel(int *p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+
+struct NoAllocDevice {
+    bool ready = false;
+    void *ptr = nullptr;               // this device's address of `scratch`
+    hipEvent_t uploaded = nullptr;     // recorded after the last table upload
+    bool recorded = false;             // `uploaded` has been recorded
+    std::vector<unsigned char> staging;  // host copy of the last tables
+};
+
+std::mutex g_noalloc_mu;
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+}  // namespace
+
+extern "C" size_t BMI_decode_tiles_noalloc_capacity(void) { return kNoAllocScratchBytes; }
+
+extern "C" int BMI_decode_tiles_noalloc(const BmiTileDesc *tiles, int ntiles,
+                                        const BmiTuning *tuning, StreamId stream)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices)
+        return bmi_fail(BMI_E_NODEVICE, "decode_tiles_noalloc: no current device");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    NoAllocDevice &D = g_noalloc[dev];
+    if (!D.ready) {
+        // once per device, with that device current: the array's device
+        // address (the symbol's host-side name is not a device address)
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return bmi_fail(BMI_E_HIP, "decode_tiles_noalloc: scratch address: %s",
+                            hipGetErrorString(e));
+        if (hipEventCreateWithFlags(&D.uploaded, hipEventDisableTiming) != hipSuccess)
+            return bmi_fail(BMI_E_HIP, "decode_tiles_noalloc: event");
+        D.ptr = p;
+        D.ready = true;
+    }
+    BmiBatchPlan *P = nullptr;
+    int rc = bmi_plan_build(&P, tiles, ntiles, tuning);
+    if (rc) return rc;
+    if (P->device_bytes > kNoAllocScratchBytes) {
+        const size_t need = P->device_bytes;
+        BMI_batch_plan_destroy(P);
+        return bmi_fail(BMI_E_NOMEM, "decode_tiles_noalloc: %zu bytes of scratch needed, %zu available",
+                        need, kNoAllocScratchBytes);
+    }
+    plan_place(P, D.ptr);
+    const hipStream_t s = (hipStream_t)stream;
+    const size_t jb = sizeof(LevelJob) * P->jobs.size();
+    const size_t ib = sizeof(WorkItem) * P->items.size();
+    const int nc = (int)P->jobs.size() + 3;
+    // the previous call's upload must have read the staging copy
+    if (D.recorded && hipEventSynchronize(D.uploaded) != hipSuccess) {
+        BMI_batch_plan_destroy(P);
+        return bmi_fail(BMI_E_HIP, "decode_tiles_noalloc: previous upload");
+    }
+    D.staging.resize(align256(jb) + ib);
+    memcpy(D.staging.data(), P->jobs.data(), jb);
+    memcpy(D.staging.data() + align256(jb), P->items.data(), ib);
+    (void)hipGetLastError();
+    bool ok = hipMemcpyAsync(P->d_jobs, D.staging.data(), align256(jb) + ib, hipMemcpyHostToDevice,
+                             s) == hipSuccess;
+    ok = ok && hipEventRecord(D.uploaded, s) == hipSuccess;
+    D.recorded = D.recorded || ok;
+    // the array keeps the previous call's contents: the persistent queue's
+    // counters must start at zero (a fill kernel, not a memset node, so the
+    // call can be captured)
+    if (ok) {
+        hipLaunchKernelGGL(zero_ints_kernel, dim3((nc + 255) / 256), dim3(256), 0, s,
+                           P->d_counters, nc);
+        ok = hipGetLastError() == hipSuccess;
+    }
+    if (!ok) {
+        BMI_batch_plan_destroy(P);
+        return bmi_fail(BMI_E_HIP, "decode_tiles_noalloc: table upload");
+    }
+    rc = plan_enqueue(P, s);
+    BMI_batch_plan_destroy(P);  // host tables only: the memory is the array's
+    return rc;
 }
