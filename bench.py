@@ -5,12 +5,17 @@ coefficients sit in HBM -- 5-level irreversible 9/7 inverse DWT on float32
 subbands (LL ~ N(128,40), detail Laplace(4)*2^-level), inverse ICT, level
 shift, clamp, packed 24-bit BGR -- with the fused engine, one launch per
 level step, timed with direct launches and with a hipGraph replay (the
-faster is reported).  Prints one JSON line.
+faster is reported).  The timed steps rotate over several buffer sets
+(coefficients, scratch, output) whose total exceeds the 256 MB last-level
+cache, so every step reads its data from HBM; the same steps on one set are
+reported beside it.  One timed output is compared with the oracle before
+anything is printed.  Prints one JSON line.
 
 Also measured, beside `value`: the plain reference-shaped GPU path (the
 drop-in BMI_comp_idwt per component + BMI_tile_ICT, 2 launches per level),
 the dominant kernel against HBM roofline, the 256-tile batch, the request
-server, tier 1 and the file-to-pixels path, and the CPU oracle on the host.
+server, tier 1 and the file-to-pixels path, and the multithreaded CPU
+restatement of the reference back end on the host (all cores and one).
 """
 from __future__ import annotations
 
@@ -49,30 +54,49 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(planes: np.ndarray, w: int, h: int, budget_s: float = 10.0):
-    """The CPU oracle (scalar C restatement of the reference CPU back end)
-    decoding the same image: one host thread per component for the
-    multi-level IDWT, then the inverse ICT and pack, repeated until about
-    budget_s of wall time."""
-    from concurrent.futures import ThreadPoolExecutor
+def _cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(planes: np.ndarray, w: int, h: int, expect: np.ndarray, budget_s: float = 10.0):
+    """The multithreaded CPU restatement of the reference CPU back end
+    (oracle/cpu_pipeline.c: rows / column bands / row bands over the
+    threads, -O3 x86-64-v3, bit-identical to the scalar oracle) decoding the
+    same image, on all host cores of the GPU's share (at most 16) and on
+    one core, each repeated for about budget_s of wall time.  Its output is
+    compared with the oracle's before it is reported."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle  # checker / CPU baseline only
-    oracle.lib()
-    ncomp = planes.shape[0]
-    threads = min(ncomp, os.cpu_count() or 1)
-    reps, t0 = 0, time.perf_counter()
-    with ThreadPoolExecutor(threads) as ex:
+    nproc = os.cpu_count() or 1
+    threads = min(16, nproc)
+    work = np.empty(planes.shape, np.float32)
+    out = np.empty((h, 3 * w), np.uint8)
+    res = {}
+    for name, t in (("all", threads), ("one", 1)):
+        reps, t0 = 0, time.perf_counter()
         while True:
-            recon = list(ex.map(lambda c: oracle.idwt97(planes[c], 5), range(ncomp)))
-            oracle.ict_pack(recon[0], recon[1], recon[2], 8, 2)
+            oracle.cpu_decode97_ict(planes, 5, t, out=out, work=work)
             reps += 1
             el = time.perf_counter() - t0
-            if el > budget_s or reps >= 30:
+            if el > budget_s or reps >= 200:
                 break
-    mpix = reps * w * h / el / 1e6
+        res[name] = (reps, el, w * h * reps / el / 1e6)
+    assert np.array_equal(out, expect), "CPU baseline output differs from the oracle"
+    reps, el, mpix = res["all"]
+    reps1, el1, mpix1 = res["one"]
     return {"value": round(mpix, 3), "unit": "Mpixel/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} full {w}x{h}x3 9/7 5-level IDWT + ICT decodes by the scalar C "
-                      f"oracle, one host thread per component ({el:.1f} s)"}
+            "nproc": nproc, "cpu_model": _cpu_model(),
+            "one_core": {"value": round(mpix1, 3), "unit": "Mpixel/s", "cores": 1,
+                         "sample": f"{reps1} decodes in {el1:.1f} s"},
+            "sample": f"{reps} full {w}x{h}x3 9/7 5-level IDWT + ICT decodes by the "
+                      f"multithreaded CPU restatement of the reference back end on {threads} "
+                      f"threads ({el:.1f} s); output equal to the scalar oracle's"}
 
 
 def _device_planes(torch, dev, w, h, filt, ncomp, gen):
@@ -404,6 +428,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--width", type=int, default=4096)
     ap.add_argument("--height", type=int, default=2160)
+    ap.add_argument("--sets", type=int, default=4,
+                    help="coefficient / scratch / output sets the timed steps rotate over "
+                         "(4 x 186 MB: more than the 256 MB last-level cache)")
+    ap.add_argument("--legs", default="single,plain,batched,serving,t1,cpu",
+                    help="comma list of legs to run (profiling runs pick one)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
     ap.add_argument("--no-graph", action="store_true")
@@ -420,6 +449,11 @@ def main() -> int:
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 256 MiB device copy, the reference for scaling HBM counters")
     args = ap.parse_args()
+    legs = set(x for x in args.legs.split(",") if x)
+    for flag, leg in (("no_cpu", "cpu"), ("no_plain", "plain"), ("no_batched", "batched"),
+                      ("no_serving", "serving"), ("no_t1", "t1")):
+        if getattr(args, flag):
+            legs.discard(leg)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the library decodes on a single MI355X "
@@ -433,111 +467,172 @@ def main() -> int:
     wl = importlib.import_module("jpeg2000_decoder_amd.workloads")
     assert torch.cuda.is_available(), "bench needs a HIP device"
     dev = torch.device("cuda:0")
-    W, H, L = args.width, args.height, 5
-
-    log("generating config: %dx%dx3 9/7 L=%d" % (W, H, L))
-    planes = wl.tile_planes(W, H, L, "97", 3, seed=0)
-    coef = [torch.from_numpy(planes[c].view(np.int32).copy()).to(dev) for c in range(3)]
-    out = torch.zeros((H, W * 3), dtype=torch.uint8, device=dev)
-    desc = pkg.tile_desc(coef_ptrs=[t.data_ptr() for t in coef], coef_pitch=W, width=W, height=H,
-                         levels=L, method=pkg.W9X7, ncomp=3, coef_kind=pkg.COEF_FLOAT,
-                         bit_depth=8, out_format=pkg.FMT_24BITS_RGB, out_ptr=out.data_ptr(),
-                         out_pitch_px=W)
-    tuning = pkg.BmiTuning(args.seg_rows, args.waves_per_group, args.groups_per_cu, args.persistent,
-                           args.min_waves)
-    plan = pkg.BatchPlan([desc], tuning)
     stream = torch.cuda.current_stream(dev)
     sh = stream.cuda_stream
-
-    log(f"fused plan: {plan.num_items} work items in {plan.num_launches} launches, "
-        f"scratch {plan.scratch_bytes / 2**20:.1f} MiB")
-
-    def timed(step):
-        for _ in range(args.warmup):
-            step()
-        torch.cuda.synchronize()
-        assert plan.error_flag() == 0, "fused kernel reported a dependency timeout"
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        ev0.record(stream)
-        for _ in range(args.steps):
-            step()
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        return ev0.elapsed_time(ev1) / args.steps, time.perf_counter() - t0
-
-    # the same K steps with launches enqueued directly and with the captured
-    # hipGraph replayed; the faster is the reported mode
-    modes = {"direct launches": lambda: plan.execute(sh)}
-    if not args.no_graph:
-        modes["hipGraph"] = lambda: plan.execute_graph(sh)
-    mode_ms = {}
-    for name, fn in modes.items():
-        mode_ms[name] = timed(fn)
-        log(f"fused ({name}): {mode_ms[name][0] * 1e3:.1f} us/step")
-    launch_mode = min(mode_ms, key=lambda k: mode_ms[k][0])
-    ms_step, t_wall = mode_ms[launch_mode]
-    mpix = W * H / (ms_step * 1e-3) / 1e6
-    log(f"fused: {ms_step * 1e3:.1f} us/step, {mpix:.0f} Mpixel/s (wall {t_wall:.3f} s)")
-
-    # Dominant kernel: the final level (multi-component lockstep, colour and
-    # pack).  Time it alone with events on its own stream, after the coarser
-    # levels of the same image have run.
-    nl = plan.num_launches
-    k_ms = []
-    for it in range(args.warmup + args.steps):
-        plan.execute_launches(0, nl - 1, sh)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        plan.execute_launches(nl - 1, 1, sh)
-        e1.record(stream)
-        if it >= args.warmup:
-            k_ms.append((e0, e1))
-    torch.cuda.synchronize()
-    k_avg = float(np.mean([a.elapsed_time(b) for a, b in k_ms]))
-    # algorithmic bytes of that kernel per launch: it reads every sample of
-    # the full-resolution Mallat planes once (the LL quadrant from the
-    # previous level, the detail bands from the coefficients: W*H*3 words)
-    # and writes every 24-bit output pixel once
-    k_bytes = W * H * 3 * 4 + W * H * 3
-    k_gbs = k_bytes / (k_avg * 1e-3) / 1e9
-    # HBM traffic of that kernel per launch, from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE passes (scaled by a calibration copy)
-    traffic = None
-    prof = ROOT / "profiles" / "fused_idwt97_ict_4096x2160x3.json"
-    if prof.exists() and (W, H) == (4096, 2160):
-        pj = json.loads(prof.read_text())
-        for kk in pj["kernels"]:
-            if "fused_step_kernel<41>" in kk["kernel"]:
-                traffic = int(kk["read_bytes"] + kk["write_bytes"])
-    roofline = {"bound": "hbm", "achieved": round(k_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_source": "profiles/fused_idwt97_ict_4096x2160x3.json" if traffic else None,
-                "kernel": "fused_step_kernel<41> (final level: 9/7 + ICT + 24-bit pack)",
-                "kernel_us": round(k_avg * 1e3, 2), "algorithmic_bytes_per_launch": k_bytes}
-    # whole decode: compulsory bytes of the job (coefficients in, pixels out)
+    W, H, L = args.width, args.height, 5
     compulsory = W * H * 3 * 4 + W * H * 3
-    pipeline_gbs = compulsory / (ms_step * 1e-3) / 1e9
-    log(f"final-level kernel: {k_avg * 1e3:.1f} us, {k_gbs:.0f} GB/s; whole decode "
-        f"{pipeline_gbs:.0f} GB/s compulsory")
+    tuning = pkg.BmiTuning(args.seg_rows, args.waves_per_group, args.groups_per_cu, args.persistent,
+                           args.min_waves, 0)
+    rec_single = {}
+    roofline = None
+    planes = None
+    expect = None
+    mpix = ms_step = None
+    plans = []
+    if "single" in legs:
+        # set 0 is the workload itself (seed 0), checked against the oracle;
+        # sets 1.. have the same distributions (seeds 1..) so that the timed
+        # steps can rotate through more memory than the last-level cache holds
+        nsets = max(1, args.sets)
+        log("generating config: %dx%dx3 9/7 L=%d, %d buffer sets" % (W, H, L, nsets))
+        sets = []
+        for k in range(nsets):
+            pl = wl.tile_planes(W, H, L, "97", 3, seed=k)
+            if k == 0:
+                planes = pl
+            coef = [torch.from_numpy(pl[c].view(np.int32).copy()).to(dev) for c in range(3)]
+            out = torch.zeros((H, W * 3), dtype=torch.uint8, device=dev)
+            desc = pkg.tile_desc(coef_ptrs=[t.data_ptr() for t in coef], coef_pitch=W, width=W,
+                                 height=H, levels=L, method=pkg.W9X7, ncomp=3,
+                                 coef_kind=pkg.COEF_FLOAT, bit_depth=8,
+                                 out_format=pkg.FMT_24BITS_RGB, out_ptr=out.data_ptr(),
+                                 out_pitch_px=W)
+            plan = pkg.BatchPlan([desc], tuning)
+            sets.append(dict(coef=coef, out=out, plan=plan))
+            plans.append(plan)
+        plan = sets[0]["plan"]
+        set_bytes = compulsory + plan.scratch_bytes
+        log(f"fused plan: {plan.num_items} work items in {plan.num_launches} launches, "
+            f"scratch {plan.scratch_bytes / 2**20:.1f} MiB; {set_bytes / 2**20:.0f} MiB per set")
 
-    if args.calib_copy:
-        a = torch.empty(64 << 20, dtype=torch.int32, device=dev)
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log("calibration copy: 3 x 268435456 bytes")
+        def timed(step, rotate):
+            n = len(sets) if rotate else 1
+            for i in range(args.warmup):
+                step(sets[i % n])
+            torch.cuda.synchronize()
+            for st in sets:
+                assert st["plan"].error_flag() == 0, "fused kernel reported a dependency timeout"
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            ev0.record(stream)
+            for i in range(args.steps):
+                step(sets[i % n])
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            return ev0.elapsed_time(ev1) / args.steps, time.perf_counter() - t0
+
+        # the same K steps with launches enqueued directly and with the captured
+        # hipGraph replayed, on one buffer set (its data may stay in the
+        # last-level cache between steps) and rotating over all sets (each
+        # step's data was last touched `sets` steps earlier)
+        modes = {"direct launches": lambda st: st["plan"].execute(sh)}
+        if not args.no_graph:
+            modes["hipGraph"] = lambda st: st["plan"].execute_graph(sh)
+        mode_ms = {}
+        for rotate in (False, True):
+            for name, fn in modes.items():
+                key = f"{name}, {'rotating over %d sets' % len(sets) if rotate else 'one set'}"
+                mode_ms[key] = (timed(fn, rotate), rotate, name)
+                log(f"fused ({key}): {mode_ms[key][0][0] * 1e3:.1f} us/step")
+        # reported: the faster launch mode over rotating sets (data from HBM)
+        rot = {k: v for k, v in mode_ms.items() if v[1]}
+        launch_key = min(rot, key=lambda k: rot[k][0][0])
+        (ms_step, t_wall), _, launch_mode = rot[launch_key]
+        one = {k: v for k, v in mode_ms.items() if not v[1]}
+        one_key = min(one, key=lambda k: one[k][0][0])
+        ms_one = one[one_key][0][0]
+        mpix = W * H / (ms_step * 1e-3) / 1e6
+        log(f"fused: {ms_step * 1e3:.1f} us/step rotating, {ms_one * 1e3:.1f} us/step on one set, "
+            f"{mpix:.0f} Mpixel/s (wall {t_wall:.3f} s)")
+
+        # Dominant kernel: the final level (multi-component lockstep, colour
+        # and pack), timed alone with events on its own stream after the
+        # coarser levels of the same image; on one set and rotating.
+        nl = plan.num_launches
+        k_avg = {}
+        for rotate in (False, True):
+            k_ms = []
+            for it in range(args.warmup + args.steps):
+                st = sets[it % len(sets)] if rotate else sets[0]
+                st["plan"].execute_launches(0, nl - 1, sh)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                st["plan"].execute_launches(nl - 1, 1, sh)
+                e1.record(stream)
+                if it >= args.warmup:
+                    k_ms.append((e0, e1))
+            torch.cuda.synchronize()
+            k_avg[rotate] = float(np.mean([a.elapsed_time(b) for a, b in k_ms]))
+        # algorithmic bytes of that kernel per launch: it reads every sample of
+        # the full-resolution Mallat planes once (the LL quadrant from the
+        # previous level, the detail bands from the coefficients: W*H*3 words)
+        # and writes every 24-bit output pixel once
+        k_bytes = W * H * 3 * 4 + W * H * 3
+        k_gbs = k_bytes / (k_avg[True] * 1e-3) / 1e9
+        k_gbs_one = k_bytes / (k_avg[False] * 1e-3) / 1e9
+        # HBM traffic of that kernel per launch, from the committed rocprofv3
+        # FETCH_SIZE / WRITE_SIZE passes (scaled by a calibration copy)
+        traffic = None
+        prof = ROOT / "profiles" / "fused_idwt97_ict_4096x2160x3.json"
+        if prof.exists() and (W, H) == (4096, 2160):
+            pj = json.loads(prof.read_text())
+            for kk in pj["kernels"]:
+                if kk.get("final_level") or "fused_step_kernel<41>" in kk["kernel"]:
+                    traffic = int(kk["read_bytes"] + kk["write_bytes"])
+        roofline = {"bound": "hbm", "achieved": round(k_gbs, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "traffic_source": "profiles/fused_idwt97_ict_4096x2160x3.json" if traffic else None,
+                    "kernel": "final level step: 9/7 + ICT + 24-bit pack",
+                    "kernel_us": round(k_avg[True] * 1e3, 2),
+                    "kernel_us_one_set": round(k_avg[False] * 1e3, 2),
+                    "achieved_one_set": round(k_gbs_one, 1),
+                    "algorithmic_bytes_per_launch": k_bytes,
+                    "timing": f"events around the launch on its stream, rotating over {len(sets)} "
+                              "buffer sets (achieved) and on one set (achieved_one_set)"}
+        pipeline_gbs = compulsory / (ms_step * 1e-3) / 1e9
+        log(f"final-level kernel: {k_avg[True] * 1e3:.1f} us rotating ({k_gbs:.0f} GB/s), "
+            f"{k_avg[False] * 1e3:.1f} us one set; whole decode {pipeline_gbs:.0f} GB/s compulsory")
+
+        # parity of a timed output: set 0's pixels from the last timed step
+        # against the oracle's decode of the same coefficients
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle  # checker only
+        expect = oracle.decode_tile(planes, levels=L, filt="97", out_format=2)[1]
+        got = sets[0]["out"].cpu().numpy()
+        parity = bool(np.array_equal(got, expect))
+        log(f"timed output equals the oracle: {parity}")
+        assert parity, "GPU output of the timed decode differs from the oracle"
+        rec_single = {"us_per_step_by_launch_mode": {k: round(v[0][0] * 1e3, 2)
+                                                     for k, v in mode_ms.items()},
+                      "us_per_step_one_set": round(ms_one * 1e3, 2),
+                      "Mpixel_s_one_set": round(W * H / (ms_one * 1e-3) / 1e6, 1),
+                      "buffer_sets": len(sets), "bytes_per_set": set_bytes,
+                      "work_items": plan.num_items, "launches": plan.num_launches,
+                      "compulsory_bytes": compulsory,
+                      "decode_GBs_compulsory": round(pipeline_gbs, 1),
+                      "decode_frac_of_peak": round(pipeline_gbs / HBM_PEAK_GBS, 4),
+                      "decode_frac_of_peak_one_set": round(compulsory / (ms_one * 1e-3) / 1e9 /
+                                                           HBM_PEAK_GBS, 4),
+                      "timed_output_equals_oracle": parity}
+        if args.calib_copy:
+            a = torch.empty(64 << 20, dtype=torch.int32, device=dev)
+            b = torch.empty_like(a)
+            for _ in range(3):
+                b.copy_(a)
+            torch.cuda.synchronize()
+            log("calibration copy: 3 x 268435456 bytes")
 
     plain = None
-    if not args.no_plain:
+    if "plain" in legs and "single" in legs:
         log("plain path (BMI_comp_idwt x3 + BMI_tile_ICT)")
+        coef = sets[0]["coef"]
         work = [torch.empty_like(t) for t in coef]
         temp = torch.empty(W * H + 64, dtype=torch.int32, device=dev)
+        pout = torch.zeros((H, W * 3), dtype=torch.uint8, device=dev)
         pdesc = pkg.tile_desc(coef_ptrs=[t.data_ptr() for t in work], coef_pitch=W, width=W,
                               height=H, levels=L, method=pkg.W9X7, ncomp=3,
                               coef_kind=pkg.COEF_FLOAT, bit_depth=8, out_format=pkg.FMT_24BITS_RGB,
-                              out_ptr=out.data_ptr(), out_pitch_px=W)
+                              out_ptr=pout.data_ptr(), out_pitch_px=W)
         times = []
         for it in range(args.warmup + args.steps):
             for a, b in zip(work, coef):
@@ -550,13 +645,16 @@ def main() -> int:
             if it >= args.warmup:
                 times.append(e0.elapsed_time(e1))
         pms = float(np.mean(times))
+        plain_ok = bool(np.array_equal(pout.cpu().numpy(), expect))
         plain = {"ms_per_step": round(pms, 4), "Mpixel_s": round(W * H / (pms * 1e-3) / 1e6, 1),
                  "fused_speedup": round(pms / ms_step, 2),
-                 "launches_per_step": 3 * (2 * L) + 1}
-        log(f"plain: {pms * 1e3:.1f} us/step, fused speedup {pms / ms_step:.2f}x")
+                 "launches_per_step": 3 * (2 * L) + 1, "output_equals_oracle": plain_ok}
+        log(f"plain: {pms * 1e3:.1f} us/step, fused speedup {pms / ms_step:.2f}x, "
+            f"equal to oracle {plain_ok}")
+        del work, temp, pout
 
     batched = None
-    if not args.no_batched:
+    if "batched" in legs:
         log("batched: 256 mixed tiles, autotuned, hipGraph")
         batched = batched_leg(pkg, wl, torch, dev, stream, max(5, args.steps // 5), 2)
         log(f"batched: {batched['ms_per_batch']} ms, {batched['Mpixel_s']} Mpixel/s, "
@@ -564,13 +662,13 @@ def main() -> int:
             f"Mpixel/s ({batched['speedup_vs_plain']}x)")
 
     serving = None
-    if not args.no_serving:
+    if "serving" in legs:
         rates = [float(r) for r in args.serve_rates.split(",") if r]
         serving = serving_leg(pkg, wl, torch, dev, rates, args.serve_requests)
 
     t1 = None
     codestream = None
-    if not args.no_t1:
+    if "t1" in legs:
         log("tier 1: GPU code-block decode + fused IDWT/ICT")
         t1 = t1_leg(pkg, wl, torch, dev, stream, max(5, args.steps // 5), 2)
         log(f"t1: {t1['t1_ms']} ms ({t1['t1_Mcoef_s']} Mcoef/s), image {t1['ms_per_image']} ms")
@@ -582,10 +680,11 @@ def main() -> int:
             f"equal={codestream['output_equals_t1_leg']}")
 
     cpu = None
-    if not args.no_cpu:
-        log("cpu baseline (oracle)")
-        cpu = cpu_baseline(planes, W, H)
-        log(f"cpu: {cpu['value']} Mpixel/s")
+    if "cpu" in legs and planes is not None:
+        log("cpu baseline (multithreaded CPU restatement, all cores and one core)")
+        cpu = cpu_baseline(planes, W, H, expect)
+        log(f"cpu: {cpu['value']} Mpixel/s on {cpu['cores']} of {cpu['nproc']} cores "
+            f"({cpu['cpu_model']}); one core {cpu['one_core']['value']} Mpixel/s")
         if t1 is not None:
             cpu["t1"] = t1_cpu_baseline(t1["_img"])
             log(f"cpu t1: {cpu['t1']['value']} Mcoef/s")
@@ -595,26 +694,22 @@ def main() -> int:
 
     rec = {
         "metric": METRIC,
-        "value": round(mpix, 1),
+        "value": round(mpix, 1) if mpix else None,
         "unit": "Mpixel/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(ms_step, 4) if ms_step else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{W}x{H}x3 9/7 5-level IDWT + inverse ICT -> 24-bit BGR, "
-                               "coefficients resident in HBM, fused engine "
-                               f"({'persistent queue' if args.persistent else 'one launch per level step'}"
-                               f", {launch_mode})",
-                   "us_per_step_by_launch_mode": {k: round(v[0] * 1e3, 2) for k, v in mode_ms.items()},
-                   "work_items": plan.num_items, "launches": plan.num_launches,
-                   "compulsory_bytes": compulsory,
-                   "decode_GBs_compulsory": round(pipeline_gbs, 1),
-                   "decode_frac_of_peak": round(pipeline_gbs / HBM_PEAK_GBS, 4)},
+        "config": dict({"workload": f"{W}x{H}x3 9/7 5-level IDWT + inverse ICT -> 24-bit BGR, "
+                                    "coefficients resident in HBM, fused engine "
+                                    f"({'persistent queue' if args.persistent else 'one launch per level step'}"
+                                    f", {launch_mode if mpix else '-'}), steps rotating over "
+                                    f"{args.sets} buffer sets"}, **rec_single),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_gpu": plain,
@@ -623,7 +718,8 @@ def main() -> int:
         "t1": t1,
         "codestream": codestream,
     }
-    plan.close()
+    for p in plans:
+        p.close()
     print(json.dumps(rec), flush=True)
     return 0
 

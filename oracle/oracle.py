@@ -286,3 +286,49 @@ def t1_zc(nb9, orient: int) -> int:
 def t1_sc(nb9) -> int:
     a = np.asarray(nb9, dtype=np.uint8)
     return lib().or_t1_sc(_p(a))
+
+
+# ------------------------------------------------- multithreaded CPU baseline --
+
+BASELINE_PATH = ORACLE_DIR / "libcpu_baseline.so"
+_BASE = None
+
+
+def baseline_lib() -> ctypes.CDLL:
+    """oracle/libcpu_baseline.so: the multithreaded CPU restatement of the
+    reference CPU back end's 9/7 + ICT pipeline (timed as the CPU baseline)."""
+    global _BASE
+    if _BASE is None:
+        if not BASELINE_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(BASELINE_PATH))
+        P = ctypes.c_void_p
+        i = ctypes.c_int
+        L.cpu_decode97_ict.argtypes = [P, i, i, i, i, i, P, P, P, i, i]
+        L.cpu_decode97_ict.restype = i
+        _BASE = L
+    return _BASE
+
+
+def cpu_decode97_ict(planes: np.ndarray, levels: int, threads: int, steps=None, out=None,
+                     work=None):
+    """Decode float coefficient planes (ncomp, h, w) at tile offset (0, 0):
+    9/7 synthesis then (ncomp >= 3) ICT to 24-bit BGR, on `threads` host
+    threads.  Returns (planes (ncomp, h, w) float32, bgr (h, 3w) uint8 or
+    None).  `out` / `work` may pass preallocated result arrays."""
+    a = np.ascontiguousarray(planes, dtype=np.float32)
+    nc, h, w = a.shape
+    if work is None:
+        work = np.empty((nc, h, w), np.float32)
+    if out is None and nc >= 3:
+        out = np.empty((h, 3 * w), np.uint8)
+    cptr = (ctypes.c_void_p * nc)(*[a[c].ctypes.data for c in range(nc)])
+    wptr = (ctypes.c_void_p * nc)(*[work[c].ctypes.data for c in range(nc)])
+    st = None
+    if steps is not None:
+        st = np.ascontiguousarray(steps, dtype=np.float32)
+    rc = baseline_lib().cpu_decode97_ict(
+        cptr, w, w, h, levels, nc, st.ctypes.data if st is not None else None, wptr,
+        out.ctypes.data if out is not None else None, 3 * w, threads)
+    assert rc == 0, rc
+    return work, out

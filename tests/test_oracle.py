@@ -381,3 +381,20 @@ def test_97_error_budget_vs_f64(orc, workloads):
           f"8-bit samples: max diff {d8.max()}, {int((d8 > 0).sum())} of {d8.size} differ")
     assert max_abs < 2e-3 and rms < 2e-4
     assert d8.max() <= 1
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+@pytest.mark.parametrize("shape", [(1003, 517, 5), (64, 40, 3), (97, 33, 4), (7, 5, 2)])
+def test_cpu_baseline_equals_oracle(orc, workloads, threads, shape):
+    """The multithreaded, vectorised CPU baseline (row-band / column-band
+    threading, -O3 x86-64-v3, no FP contraction) gives the scalar oracle's
+    planes and pixels bit for bit at any thread count, so the time it
+    reports is the time of the same decode."""
+    w, h, L = shape
+    planes = workloads.tile_planes(w, h, L, "97", 3, seed=w + h)
+    steps = np.stack([np.linspace(0.5 + 0.2 * c, 1.5, 3 * L + 1) for c in range(3)]).astype(np.float32)
+    got_planes, got = orc.cpu_decode97_ict(planes, L, threads, steps=steps)
+    exp_planes, exp = orc.decode_tile(planes, levels=L, filt="97", out_format=2, steps=steps)
+    for c in range(3):
+        assert np.array_equal(got_planes[c].view(np.uint32), exp_planes[c].view(np.uint32)), c
+    assert np.array_equal(got, exp)
