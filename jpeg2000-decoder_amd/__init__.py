@@ -94,6 +94,7 @@ class BmiTuning(ctypes.Structure):
         ("persistent", ctypes.c_int),
         ("min_waves_per_cu", ctypes.c_int),
         ("fixed_seg", ctypes.c_int),
+        ("fuse2", ctypes.c_int),
     ]
 
 

@@ -46,6 +46,15 @@ struct LevelJob {
     int ndeps;
     int deps[kMaxDeps];  // jobs that must be complete first
     int n_items;
+    // two-level jobs (fuse2 = 1): the level below this final level is
+    // synthesised inside the same items, into a ring of rows in LDS, and
+    // never goes to HBM.  Its output span and band origins (absolute):
+    int fuse2;
+    int ia0, ia1, ib0, ib1;
+    int ilw, ilh, icl0, ich0, irl0, irh0;
+    int ill_kind, ill_pitch;
+    const void *ill[4];  // LL source of the inner level per slot
+    float igain[4][4];   // 9/7 gains of the inner level per slot
 };
 
 struct WorkItem {

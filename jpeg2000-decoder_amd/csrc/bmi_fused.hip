@@ -433,6 +433,8 @@ struct Emitter {
 // (deeper for single-component items, which need fewer registers).
 template <int NC>
 struct Prefetch {
+    // deeper rings for several components (3 or 4 pairs in flight) made the
+    // final level slower with its data in HBM (37.6 -> 38.8 / 39.3 us)
     static constexpr int D = (NC == 1) ? 4 : 2;
 };
 
@@ -869,11 +871,302 @@ __device__ __attribute__((always_inline)) inline void thumb_item(const LevelJob 
     em.put(y, x, v);
 }
 
+// Rows of the inner level kept per wave by a two-level item: the final
+// level reads them in order except near the tile edges, where the
+// symmetric extension revisits at most four rows back.
+constexpr int kRingRows = 16;
+
+// Two-level work item (9/7): strip `strip`, row segment `seg` of the final
+// level of J, with the level below it synthesised in the same wave.  The
+// inner level runs the same pipeline as process_item over the 64 inner
+// columns this strip's LL band needs (32 or 33 column pairs plus the
+// lifting halo: lanes 2..35 of the wave) and writes each finished row to a
+// ring of kRingRows rows in this wave's LDS; the final level takes its LL
+// samples from that ring instead of a scratch plane.  The inner level runs
+// one step (one row pair in, two rows out) whenever the final level needs a
+// row it has not produced yet.  Every sample gets the operations of the
+// two-launch form in the same order, so the output is the same bit for bit;
+// the inner level's output never goes to HBM.  Segments run top-down only.
+template <int NC, int OUTK, int PL, int KD>
+__device__ __attribute__((always_inline)) inline void process_item2(
+    const LevelJob &__restrict__ J, int strip, int seg, int lane, float *__restrict__ ring)
+{
+    constexpr int FILT = 97;
+    constexpr int SK_ILL = (KD == KD_FF || KD == KD_FS) ? SK_FLOAT : (KD == KD_SS ? SK_SIGNMAG : SK_RUNTIME);
+    constexpr int SK_DET = (KD == KD_FF) ? SK_FLOAT : ((KD == KD_FS || KD == KD_SS) ? SK_SIGNMAG : SK_RUNTIME);
+    using T = float;
+    constexpr int HP = 2, OWP = 60;
+    constexpr int D = 2;  // final-level pairs in flight
+
+    // ---- final level: as process_item, without the LL loads
+    const int a0 = J.a0, a1 = J.a1, b0 = J.b0, b1 = J.b1;
+    const int lw = J.lw, lh = J.lh, rl0 = J.rl0, rh0 = J.rh0;
+    const int det_pitch = J.det_pitch;
+    const int det_float = J.det_kind == SRC_FLOAT;
+    const __attribute__((address_space(1))) uint32_t *detp[NC];
+    float gn[NC][4];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        detp[c] = gptr<uint32_t>(J.det[c]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) gn[c][j] = J.gain[c][j];
+    }
+    const int wpair0 = ((a0 & ~1) >> 1) + strip * OWP - HP;
+    const int ce = 2 * (wpair0 + lane), co = ce + 1;
+    const bool lane_out = (lane >= HP) && (lane < 64 - HP);
+    const int lo_idx = (reflect(ce, a0, a1) >> 1) - J.cl0;
+    const int hi_off = lw + (reflect(co, a0, a1) >> 1) - J.ch0;
+    const int K0 = (b0 & ~1) >> 1;
+    const int Klast = (b1 - 1) >> 1;
+    const int kout0 = K0 + seg * J.seg_pairs;
+    const int kout1 = min(Klast, kout0 + J.seg_pairs - 1);
+    const int ks = kout0 - 2, ke = kout1 + 2;
+    const int nrefl = (b1 - b0) >= 8 ? 1 : 3;
+    auto reflect_row = [&](int r) {
+#pragma unroll 1
+        for (int i = 0; i < nrefl; ++i) {
+            r = r < b0 ? 2 * b0 - r : r;
+            r = r >= b1 ? 2 * (b1 - 1) - r : r;
+        }
+        return r;
+    };
+    auto ld = [](const __attribute__((address_space(1))) uint32_t *base, uint32_t elem) {
+        return *(const __attribute__((address_space(1))) uint32_t *)(
+            (const __attribute__((address_space(1))) char *)base + (elem << 2));
+    };
+    // LL row (relative to the inner level's first row) of final pair k
+    auto ll_row = [&](int k) { return (reflect_row(2 * k) >> 1) - rl0; };
+    auto fetch = [&](int k, uint32_t (&r)[NC][3]) {
+        const int rre = reflect_row(2 * k), rro = reflect_row(2 * k + 1);
+        const uint32_t rt = (uint32_t)((rre >> 1) - rl0);
+        const uint32_t q = (uint32_t)((rro >> 1) - rh0 + lh);
+        const uint32_t dtr = rt * (uint32_t)det_pitch, dbr = q * (uint32_t)det_pitch;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            r[c][0] = ld(detp[c], dtr + (uint32_t)hi_off);
+            r[c][1] = ld(detp[c], dbr + (uint32_t)lo_idx);
+            r[c][2] = ld(detp[c], dbr + (uint32_t)hi_off);
+        }
+    };
+
+    // ---- the LDS window: inner output columns [mbase, mbase + 64)
+    // (relative to the inner level's first column); lane j of the final
+    // level reads slot lo_idx - mbase (its unreflected column is mbase + j;
+    // a reflected one of a lane that matters stays inside the window, and
+    // lanes far past the tile edge only feed halo lanes that are never
+    // stored, so their slot is clamped)
+    const int mbase = wpair0 - J.cl0;
+    const int xl = min(max(lo_idx - mbase, 0), 63);
+
+    // ---- inner level: geometry of process_item for the window's columns
+    const int ia0 = J.ia0, ia1 = J.ia1, ib0 = J.ib0, ib1 = J.ib1;
+    const int ilw = J.ilw, ilh = J.ilh, irl0 = J.irl0, irh0 = J.irh0;
+    const int ill_pitch = J.ill_pitch;
+    const int ill_float = J.ill_kind == SRC_FLOAT;
+    const __attribute__((address_space(1))) uint32_t *illp[NC];
+    float ign[NC][4];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        illp[c] = gptr<uint32_t>(J.ill[c]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ign[c][j] = J.igain[c][j];
+    }
+    const int inh = ib1 - ib0;            // inner output rows (the final level's LL rows)
+    const int W0 = ia0 + mbase;           // absolute inner column of slot 0
+    const int ipair0 = (W0 >> 1) - HP;    // floor: W0 may be negative
+    const int ice = 2 * (ipair0 + lane), ico = ice + 1;
+    const int ilo_idx = (reflect(ice, ia0, ia1) >> 1) - J.icl0;
+    const int ihi_off = ilw + (reflect(ico, ia0, ia1) >> 1) - J.ich0;
+    const int xe = ice - W0, xo = ico - W0;
+    const bool we = lane >= HP && xe >= 0 && xe < 64;
+    const bool wo = lane >= HP && xo >= 0 && xo < 64;
+    const int inrefl = (ib1 - ib0) >= 8 ? 1 : 3;
+    auto ireflect_row = [&](int r) {
+#pragma unroll 1
+        for (int i = 0; i < inrefl; ++i) {
+            r = r < ib0 ? 2 * ib0 - r : r;
+            r = r >= ib1 ? 2 * (ib1 - 1) - r : r;
+        }
+        return r;
+    };
+    auto ifetch = [&](int q, uint32_t (&r)[NC][4]) {
+        const int rre = ireflect_row(2 * q), rro = ireflect_row(2 * q + 1);
+        const uint32_t rt = (uint32_t)((rre >> 1) - irl0);
+        const uint32_t qq = (uint32_t)((rro >> 1) - irh0 + ilh);
+        const uint32_t llr = rt * (uint32_t)ill_pitch, dtr = rt * (uint32_t)det_pitch;
+        const uint32_t dbr = qq * (uint32_t)det_pitch;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            r[c][0] = ld(illp[c], llr + (uint32_t)ilo_idx);
+            r[c][1] = ld(detp[c], dtr + (uint32_t)ihi_off);
+            r[c][2] = ld(detp[c], dbr + (uint32_t)ilo_idx);
+            r[c][3] = ld(detp[c], dbr + (uint32_t)ihi_off);
+        }
+    };
+
+    // inner rows this segment reads: [rmin, rmax], relative to ib0
+    int rmin = 0x7fffffff, rmax = -1;
+#pragma unroll 1
+    for (int k = ks; k <= ke; ++k) {
+        const int r = ll_row(k);
+        rmin = min(rmin, r);
+        rmax = max(rmax, r);
+    }
+    const int qo0 = (ib0 + rmin) >> 1, qo1 = (ib0 + rmax) >> 1;  // inner pairs to emit
+    const int iqe = qo1 + 2;
+    int q = qo0 - 2;   // next inner pair to consume
+    int have = rmin;   // rows below `have` (from rmin on) are in the ring
+
+    // inner pipeline state (top-down): s0 = H_{q-1}, s1 = e3_{q-1},
+    // s2 = o4_{q-2}, s3 = e5_{q-2}; one pair of loads in flight
+    T t0[NC][2], t1[NC][2], t2[NC][2], t3[NC][2];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) t0[c][e] = t1[c][e] = t2[c][e] = t3[c][e] = 0.0f;
+    uint32_t islot[NC][4];
+    ifetch(q, islot);
+
+    auto inner_step = [&]() {
+        T smp[NC][4];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            smp[c][0] = to_sample<FILT, SK_ILL>(islot[c][0], ill_float, ign[c][0]);
+            smp[c][1] = to_sample<FILT, SK_DET>(islot[c][1], det_float, ign[c][1]);
+            smp[c][2] = to_sample<FILT, SK_DET>(islot[c][2], det_float, ign[c][2]);
+            smp[c][3] = to_sample<FILT, SK_DET>(islot[c][3], det_float, ign[c][3]);
+        }
+        ifetch(min(q + 1, iqe), islot);
+        const int po = q - 2;  // pair emitted by this step
+        const int r0 = 2 * po - ib0;  // its first row, relative
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            T tl = smp[c][0], th = smp[c][1], bl = smp[c][2], bh = smp[c][3];
+            hlift<FILT>(tl, th);
+            hlift<FILT>(bl, bh);
+            const T Lk[2] = {tl, th}, Hk[2] = {bl, bh};
+            T ev[2], od[2];
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const float e3 = lift97(Lk[e], kDelta, t0[c][e], Hk[e]);
+                const float o4 = lift97(t0[c][e], kGamma, t1[c][e], e3);
+                const float e5 = lift97(t1[c][e], kBeta, t2[c][e], o4);
+                const float o6 = lift97(t2[c][e], kAlpha, t3[c][e], e5);
+                ev[e] = t3[c][e];
+                od[e] = o6;
+                t0[c][e] = Hk[e];
+                t1[c][e] = e3;
+                t2[c][e] = o4;
+                t3[c][e] = e5;
+            }
+            if (po >= qo0) {
+                // rows r0 (even) and r0 + 1 (odd) of the inner output
+                if (r0 >= 0 && r0 < inh) {
+                    float *row = ring + (((r0 & (kRingRows - 1)) * NC + c) << 6);
+                    if (we) row[xe] = ev[0];
+                    if (wo) row[xo] = ev[1];
+                }
+                if (r0 + 1 >= 0 && r0 + 1 < inh) {
+                    float *row = ring + ((((r0 + 1) & (kRingRows - 1)) * NC + c) << 6);
+                    if (we) row[xe] = od[0];
+                    if (wo) row[xo] = od[1];
+                }
+            }
+        }
+        if (po >= qo0) have = max(have, r0 + 2);
+        ++q;
+    };
+
+    Emitter<FILT, NC, OUTK, PL> em;
+    em.init(J);
+    const bool ce_ok = lane_out && ce >= a0 && ce < a1;
+    const bool co_ok = lane_out && co >= a0 && co < a1;
+    em.set_lanes(ce_ok, co_ok);
+    const bool full = em.full_ok();
+
+    // final-level pipeline state (top-down): s0 = H_{k-1}, s1 = e3_{k-1},
+    // s2 = o4_{k-2}, s3 = e5_{k-2}
+    T s0[NC][2], s1[NC][2], s2[NC][2], s3[NC][2];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) s0[c][e] = s1[c][e] = s2[c][e] = s3[c][e] = 0.0f;
+
+    uint32_t slot[D][NC][3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) fetch(min(ks + d, ke), slot[d]);
+    // the LL samples of the next pair, read from the ring one pair ahead
+    // (the inner level runs as far as that pair needs)
+    float llnext[NC];
+    auto read_ll = [&](int k) {
+        const int rk = ll_row(min(k, ke));
+#pragma unroll 1
+        while (have <= rk && q <= iqe) inner_step();
+        const float *row = ring + (((rk & (kRingRows - 1)) * NC) << 6);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) llnext[c] = row[(c << 6) + xl];
+    };
+    read_ll(ks);
+    const int rounds = (ke - ks + D) / D;
+    for (int it = 0; it < rounds; ++it) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int k = ks + it * D + d;
+            T smp[NC][4];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                smp[c][0] = __fmul_rn(llnext[c], gn[c][0]);
+                smp[c][1] = to_sample<FILT, SK_DET>(slot[d][c][0], det_float, gn[c][1]);
+                smp[c][2] = to_sample<FILT, SK_DET>(slot[d][c][1], det_float, gn[c][2]);
+                smp[c][3] = to_sample<FILT, SK_DET>(slot[d][c][2], det_float, gn[c][3]);
+            }
+            fetch(min(k + D, ke), slot[d]);
+            read_ll(k + 1);
+
+            T evn[2][NC], odd[2][NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                T tl = smp[c][0], th = smp[c][1], bl = smp[c][2], bh = smp[c][3];
+                hlift<FILT>(tl, th);
+                hlift<FILT>(bl, bh);
+                const T Lk[2] = {tl, th}, Hk[2] = {bl, bh};
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const float e3 = lift97(Lk[e], kDelta, s0[c][e], Hk[e]);
+                    const float o4 = lift97(s0[c][e], kGamma, s1[c][e], e3);
+                    const float e5 = lift97(s1[c][e], kBeta, s2[c][e], o4);
+                    const float o6 = lift97(s2[c][e], kAlpha, s3[c][e], e5);
+                    evn[e][c] = s3[c][e];
+                    odd[e][c] = o6;
+                    s0[c][e] = Hk[e];
+                    s1[c][e] = e3;
+                    s2[c][e] = o4;
+                    s3[c][e] = e5;
+                }
+            }
+            const int ko = k - 2;
+            if (ko >= kout0 && ko <= kout1) {
+                if (full && 2 * ko >= b0 && 2 * ko + 1 < b1) {
+                    em.put2_full(2 * ko, ce, evn[0], evn[1], ce_ok);
+                    em.put2_full(2 * ko + 1, ce, odd[0], odd[1], ce_ok);
+                } else {
+                    if (2 * ko >= b0 && 2 * ko < b1) em.put2(2 * ko, ce, evn[0], evn[1], ce_ok, co_ok);
+                    if (2 * ko + 1 >= b0 && 2 * ko + 1 < b1)
+                        em.put2(2 * ko + 1, ce, odd[0], odd[1], ce_ok, co_ok);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
 // Kernel variants: V = (filt == 97) * 32 + sub * 4 + kd.
 // sub: 0 plane (1 comp), 1 merge (1 comp), 2..5 RGB of layout
 // PL_BGR8..PL_BGRA16 (3 comps), 6 merge (2-3 comps), 7 merge (4 comps).
 // kd: coefficient encodings (9/7 planes and RGB only; KD_ANY otherwise).
 // Thumbnail jobs (lev < 0): V = 64 + (filt == 97) * 8 + sub.
+// Two-level jobs (9/7 only): V = 96 + sub * 4 + kd, kd from the inner
+// level's LL encoding and the detail bands'.
 __host__ __device__ inline int variant_of(const LevelJob &J)
 {
     int sub;
@@ -883,10 +1176,35 @@ __host__ __device__ inline int variant_of(const LevelJob &J)
     if (J.lev < 0) return 64 + (J.filt == 97 ? 8 : 0) + sub;
     int kd = KD_ANY;
     if (J.filt == 97 && (sub == 0 || (sub >= 2 && sub <= 5))) {
-        const bool llf = J.ll_kind == SRC_FLOAT, detf = J.det_kind == SRC_FLOAT;
+        const bool llf = (J.fuse2 ? J.ill_kind : J.ll_kind) == SRC_FLOAT, detf = J.det_kind == SRC_FLOAT;
         kd = (llf && detf) ? KD_FF : (llf ? KD_FS : (detf ? KD_ANY : KD_SS));
     }
+    if (J.fuse2) return 96 + sub * 4 + kd;
     return (J.filt == 97 ? 32 : 0) + sub * 4 + kd;
+}
+
+// the two-level variants instantiated, as X(V); a final level whose
+// variant is not listed keeps the two-launch form
+#define BMI_FOR_EACH_FUSED_VARIANT(X)                                                       \
+    X(96) X(97) X(98) X(99) X(100) X(104) X(105) X(106) X(107) X(109) X(113) X(120)
+
+__host__ __device__ inline bool fused_variant_available(int v)
+{
+    switch (v) {
+#define BMI_FUSED_CASE(V) case V:
+        BMI_FOR_EACH_FUSED_VARIANT(BMI_FUSED_CASE)
+#undef BMI_FUSED_CASE
+        return true;
+    default:
+        return false;
+    }
+}
+
+// lockstep components of a two-level variant (its LDS ring per wave holds
+// kRingRows rows of 64 floats per component)
+__host__ __device__ constexpr int fused_variant_nc(int v)
+{
+    return (((v - 96) >> 2) & 7) <= 1 ? 1 : ((((v - 96) >> 2) & 7) == 7 ? 4 : 3);
 }
 
 template <int V>
@@ -1028,6 +1346,40 @@ __global__ void __launch_bounds__(256) fused_step_kernel(const LevelJob *__restr
     run_variant<V>(J, local % J.nstrips, local / J.nstrips, lane);
 }
 
+// level-step mode, two-level jobs: the inner level's rows live in the
+// dynamic LDS of each wave (kRingRows x components x 64 floats)
+// (Capping it at 128 VGPRs for four waves per SIMD spills and runs about
+// 1.5x slower.)
+template <int V>
+__global__ void __launch_bounds__(256) fused2_step_kernel(const LevelJob *__restrict__ jobs,
+                                                          const WorkItem *__restrict__ items,
+                                                          int first, int count, int remap)
+{
+    extern __shared__ float bmi_ring_lds[];
+    constexpr int SUB = ((V - 96) >> 2) & 7;
+    constexpr int KD = (V - 96) & 3;
+    constexpr int NC = fused_variant_nc(V);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    int g = blockIdx.x;
+    if (remap) {
+        const int G = gridDim.x, x = g & 7, j = g >> 3, q = G >> 3, r = G & 7;
+        g = x * q + min(x, r) + j;
+    }
+    const int i = g * (blockDim.x >> 6) + wave;
+    if (i >= count) return;
+    const int job = __builtin_amdgcn_readfirstlane(items[first + i].job);
+    const int local = __builtin_amdgcn_readfirstlane(items[first + i].local);
+    const LevelJob &__restrict__ J = jobs[job];
+    float *ring = bmi_ring_lds + wave * (kRingRows * NC * 64);
+    const int strip = local % J.nstrips, seg = local / J.nstrips;
+    if constexpr (SUB == 0) process_item2<1, K_PLANE, PL_ANY, KD>(J, strip, seg, lane, ring);
+    else if constexpr (SUB == 1) process_item2<1, K_MERGE, PL_ANY, KD>(J, strip, seg, lane, ring);
+    else if constexpr (SUB >= 2 && SUB <= 5) process_item2<3, K_RGB, SUB - 2, KD>(J, strip, seg, lane, ring);
+    else if constexpr (SUB == 6) process_item2<3, K_MERGE, PL_ANY, KD>(J, strip, seg, lane, ring);
+    else process_item2<4, K_MERGE, PL_ANY, KD>(J, strip, seg, lane, ring);
+}
+
 // level-step mode with all kernel variants of a step in one launch (each
 // wave dispatches on its item's variant): one launch per step instead of
 // one per step and variant, at the register budget of the largest variant
@@ -1069,8 +1421,9 @@ struct BmiBatchPlan {
     std::vector<Launch> launches;  // level-step mode: per step and variant
     std::vector<int> job_tile;                // tile of each job
     struct {
-        int seg_rows, waves_per_group, groups_per_cu, persistent, min_waves_per_cu, fixed_seg;
+        int seg_rows, waves_per_group, groups_per_cu, persistent, min_waves_per_cu, fixed_seg, fuse2;
     } tuning;
+    std::vector<char> tile_fused;  // per tile: its last two levels run as one two-level step
     LevelJob *d_jobs = nullptr;
     WorkItem *d_items = nullptr;
     int *d_counters = nullptr;  // [njobs] completion counters; [njobs] queue head,
@@ -1167,6 +1520,37 @@ void needed_comps(const BmiTileDesc &t, int ok, int *list, int *n)
 
 }  // namespace
 
+// A tile runs its last two synthesis levels as one two-level step when
+// both levels are 9/7, at least two samples in each direction, and a
+// two-level kernel exists for its final level's variant.
+static bool tile_fuses(const BmiTileDesc &t)
+{
+    if (t.method != Ckernels_W9X7) return false;
+    const int L = t.levels;
+    const int Ls = (t.top_level < 0) ? L : t.top_level;
+    if (Ls < 2) return false;
+    for (int lev = Ls - 2; lev < Ls; ++lev) {
+        int a0, a1, b0, b1;
+        level_span(t.x0, t.width, L, lev, &a0, &a1);
+        level_span(t.y0, t.height, L, lev, &b0, &b1);
+        if (a1 - a0 < 2 || b1 - b0 < 2) return false;
+    }
+    const int ok = out_kind_for(t);
+    int comps[4], n = 0;
+    needed_comps(t, ok, comps, &n);
+    LevelJob J;
+    memset(&J, 0, sizeof(J));
+    J.filt = 97;
+    J.lev = Ls - 1;
+    J.out_kind = ok;
+    J.fmt = t.out_format;
+    J.nc = (ok == OUT_PLANE) ? 1 : n;
+    J.fuse2 = 1;
+    J.det_kind = t.coef_kind == BMI_COEF_FLOAT ? SRC_FLOAT : SRC_SIGNMAG;
+    J.ill_kind = (Ls - 2 == 0) ? J.det_kind : SRC_FLOAT;
+    return fused_variant_available(variant_of(J));
+}
+
 static int default_seg_rows(const BmiTuning *t) { return (t && t->seg_rows > 0) ? t->seg_rows : 32; }
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -1194,6 +1578,7 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
     P->tuning.persistent = tuning ? tuning->persistent : 0;
     P->tuning.min_waves_per_cu = (tuning && tuning->min_waves_per_cu > 0) ? tuning->min_waves_per_cu : 4;
     P->tuning.fixed_seg = (tuning && tuning->fixed_seg) ? 1 : 0;
+    P->tuning.fuse2 = (tuning && tuning->fuse2 < 0) ? -1 : 0;
     if (P->tuning.waves_per_group > 4) P->tuning.waves_per_group = 4;
     if (hipGetDevice(&P->device) != hipSuccess) P->device = 0;
     {
@@ -1204,8 +1589,14 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
             P->num_cus = cus;
     }
 
+    // which tiles run their last two levels as one two-level step
+    P->tile_fused.assign(ntiles, 0);
+    const bool fuse_mode = P->tuning.fuse2 >= 0 && P->tuning.persistent != 1 && P->tuning.persistent != 2;
+    for (int ti = 0; ti < ntiles && fuse_mode; ++ti) P->tile_fused[ti] = tile_fuses(tiles[ti]) ? 1 : 0;
+
     // scratch layout: per tile, per needed component, two planes sized for
-    // the largest intermediate level
+    // the largest intermediate level written to HBM (a two-level step keeps
+    // its inner level in LDS)
     size_t total = 0;
     P->tile_scratch_off.resize(ntiles);
     P->tile_scratch_pitch.resize(ntiles);
@@ -1214,8 +1605,9 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
         const BmiTileDesc &t = tiles[ti];
         const int L = t.levels;
         const int Ls = (t.top_level < 0) ? L : t.top_level;
+        const int Lw = Ls - (P->tile_fused[ti] ? 1 : 0);
         int maxw = 0, maxh = 0;
-        for (int lev = 0; lev + 1 < Ls; ++lev) {
+        for (int lev = 0; lev + 1 < Lw; ++lev) {
             int a0, a1, b0, b1;
             level_span(t.x0, t.width, L, lev, &a0, &a1);
             level_span(t.y0, t.height, L, lev, &b0, &b1);
@@ -1309,6 +1701,8 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                 continue;
             }
             if (lev >= Ls) continue;
+            const bool fused = P->tile_fused[ti] != 0;
+            if (fused && lev == Ls - 2) continue;  // synthesised inside the final level's items
             const bool final_level = (lev == Ls - 1);
             const int ok = final_level ? out_kind_for(t) : (int)OUT_SCRATCH;
             const int filt = (t.method == Ckernels_W9X7) ? 97 : 53;
@@ -1373,9 +1767,42 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                 }
             };
             auto add_deps = [&](LevelJob &J, int c) {
-                if (lev == 0) return;
-                const int dj = job_of[ti][(lev - 1) * 4 + c];
+                const int src = lev - (fused && final_level ? 2 : 1);
+                if (src < 0) return;
+                const int dj = job_of[ti][src * 4 + c];
                 if (dj >= 0 && J.ndeps < kMaxDeps) J.deps[J.ndeps++] = dj;
+            };
+            // a two-level final job: geometry, LL source and gains of the
+            // level below (level lev - 1), which never gets a job of its own
+            auto set_fuse_geom = [&](LevelJob &J) {
+                if (!(fused && final_level)) return;
+                int ia0, ia1, ib0, ib1;
+                level_span(t.x0, t.width, L, lev - 1, &ia0, &ia1);
+                level_span(t.y0, t.height, L, lev - 1, &ib0, &ib1);
+                J.fuse2 = 1;
+                J.ia0 = ia0;
+                J.ia1 = ia1;
+                J.ib0 = ib0;
+                J.ib1 = ib1;
+                J.ilw = ceil_div2(ia1) - ceil_div2(ia0);
+                J.ilh = ceil_div2(ib1) - ceil_div2(ib0);
+                J.icl0 = ceil_div2(ia0);
+                J.ich0 = floor_div2(ia0);
+                J.irl0 = ceil_div2(ib0);
+                J.irh0 = floor_div2(ib0);
+                J.ill_kind = (lev - 1 == 0) ? J.det_kind : SRC_FLOAT;
+                J.ill_pitch = (lev - 1 == 0) ? t.coef_pitch : P->tile_scratch_pitch[ti];
+            };
+            auto fill_fuse = [&](LevelJob &J, int slot, int c) {
+                if (!(fused && final_level)) return;
+                const int il = lev - 1;
+                J.ll[slot] = nullptr;  // the final level's LL comes from LDS
+                J.ill[slot] = (il == 0) ? t.coef[c] : scratch_plane(c, (il - 1) & 1);
+                const float *st = t.steps[c];
+                J.igain[slot][0] = il == 0 ? (st[0] * kLoGain) * kLoGain : kLoGain * kLoGain;
+                J.igain[slot][1] = (st[3 * il + 1] * kHiGain) * kLoGain;
+                J.igain[slot][2] = (st[3 * il + 2] * kLoGain) * kHiGain;
+                J.igain[slot][3] = (st[3 * il + 3] * kHiGain) * kHiGain;
             };
             auto push_job = [&](const LevelJob &J, int key_comp) {
                 const int jid = (int)P->jobs.size();
@@ -1407,6 +1834,8 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                     LevelJob J = base;
                     J.nc = 1;
                     fill_comp(J, 0, c);
+                    set_fuse_geom(J);
+                    fill_fuse(J, 0, c);
                     J.dst[0] = t.planes_out[c];
                     J.dst_pitch = t.planes_pitch;
                     add_deps(J, c);
@@ -1415,8 +1844,10 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
             } else {
                 LevelJob J = base;
                 J.nc = ncomps;
+                set_fuse_geom(J);
                 for (int i = 0; i < ncomps; ++i) {
                     fill_comp(J, i, comps[i]);
+                    fill_fuse(J, i, comps[i]);
                     add_deps(J, comps[i]);
                 }
                 // a merge that checks out one component shows it as grey
@@ -1427,7 +1858,9 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                     J.comp[i] = J.comp[0];
                     J.ll[i] = J.ll[0];
                     J.det[i] = J.det[0];
+                    J.ill[i] = J.ill[0];
                     for (int g = 0; g < 4; ++g) J.gain[i][g] = J.gain[0][g];
+                    for (int g = 0; g < 4; ++g) J.igain[i][g] = J.igain[0][g];
                 }
                 push_job(J, -1);
             }
@@ -1536,17 +1969,26 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
             // dispatching kernel carries every variant's code and the
             // largest register budget; the autotuner tries both.
             {
+                // two-level launches (variants 96 and up, sorted last) keep
+                // their own kernels: the dispatching kernel does not carry them
                 int n0 = -1, nv = 0;
+                std::vector<BmiBatchPlan::Launch> two;
                 for (size_t q = 0; q < P->launches.size(); ++q)
                     if (P->launches[q].step == step) {
+                        if (P->launches[q].variant >= 96) {
+                            two.push_back(P->launches[q]);
+                            continue;
+                        }
                         if (n0 < 0) n0 = (int)q;
                         ++nv;
                     }
                 const int mode = P->tuning.persistent;
                 if (n0 >= 0 && ((mode == 0 && nv > 1) || mode == 2)) {
                     const int f = P->launches[n0].first;
+                    const int end = two.empty() ? (int)P->items.size() : two.front().first;
                     P->launches.resize(n0);
-                    P->launches.push_back({f, (int)P->items.size() - f, -1, step});
+                    P->launches.push_back({f, end - f, -1, step});
+                    for (const auto &l : two) P->launches.push_back(l);
                 }
             }
         }
@@ -1627,8 +2069,10 @@ static void plan_place(BmiBatchPlan *P, void *base)
     for (LevelJob &J : P->jobs) {
         const bool packed = (J.out_kind == OUT_RGB || J.out_kind == OUT_MERGE);
         const int nslots = packed ? 4 : 1;
-        if (J.lev > 0)
+        if (J.lev > 0 && !J.fuse2)
             for (int s = 0; s < nslots; ++s) J.ll[s] = (const char *)J.ll[s] + (uintptr_t)b;
+        if (J.fuse2 && J.lev > 1)  // the inner level reads a scratch plane
+            for (int s = 0; s < nslots; ++s) J.ill[s] = (const char *)J.ill[s] + (uintptr_t)b;
         if (J.out_kind == OUT_SCRATCH) J.dst[0] = (char *)J.dst[0] + (uintptr_t)b;
     }
 }
@@ -1690,6 +2134,21 @@ static int launch_one(const BmiBatchPlan *P, const BmiBatchPlan::Launch &l, hipS
     if (l.variant < 0) {
         hipLaunchKernelGGL(fused_step_any_kernel, grid, block, 0, s, P->d_jobs, P->d_items, l.first,
                            l.count, xcd_remap);
+        return BMI_OK;
+    }
+    if (l.variant >= 96) {
+        const size_t lds = (size_t)wpg * kRingRows * fused_variant_nc(l.variant) * 64 * sizeof(float);
+        switch (l.variant) {
+#define BMI_STEP2_CASE(V)                                                                        \
+    case V:                                                                                     \
+        hipLaunchKernelGGL((fused2_step_kernel<V>), grid, block, lds, s, P->d_jobs, P->d_items, \
+                           l.first, l.count, xcd_remap);                                        \
+        break;
+            BMI_FOR_EACH_FUSED_VARIANT(BMI_STEP2_CASE)
+#undef BMI_STEP2_CASE
+        default:
+            return bmi_fail(BMI_E_INVALID, "fused launch: no two-level kernel for variant %d", l.variant);
+        }
         return BMI_OK;
     }
     switch (l.variant) {
@@ -1853,6 +2312,7 @@ extern "C" int BMI_batch_plan_rebind(BmiBatchPlan *P, int i, const BmiTileDesc *
             const int c = J.comp[s];
             J.det[s] = t->coef[c];
             if (J.lev <= 0) J.ll[s] = t->coef[c];
+            if (J.fuse2 && J.lev == 1) J.ill[s] = t->coef[c];
             if (J.out_kind == OUT_PLANE && s < J.nc) J.dst[s] = t->planes_out[c];
         }
         if (J.out_kind == OUT_RGB || J.out_kind == OUT_MERGE) {

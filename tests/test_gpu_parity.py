@@ -227,3 +227,49 @@ def test_ict_saturation_in_mixed_batch(pkg, workloads, orc, mode):
     for i, ((_, keep), exp) in enumerate(zip(cases, exps)):
         _, got = fetch(keep, specs[i]["filt"])
         assert np.array_equal(got, exp), f"tile {i}: {diff_report(got, exp)}"
+
+
+FUSE_SHAPES = [
+    # (w, h, levels, x0, y0, seg_rows): the last two 9/7 levels in one pass
+    (4, 4, 2, 0, 0, 0), (5, 7, 2, 1, 1, 0), (9, 6, 3, 0, 1, 2), (33, 17, 3, 1, 0, 4),
+    (131, 97, 4, 3, 5, 6), (257, 131, 5, 0, 0, 0), (301, 199, 5, 1, 2, 10), (130, 257, 6, 0, 0, 38),
+    (600, 40, 3, 0, 0, 0), (40, 600, 3, 0, 0, 64),
+]
+
+
+@pytest.mark.parametrize("shape", FUSE_SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("kind", ["float", "signmag"])
+def test_two_level_final_step_matches_oracle(pkg, workloads, orc, shape, kind):
+    """The final two 9/7 levels synthesised in one pass (the lower one in an
+    LDS ring of rows): planes and 24-bit BGR equal the oracle and the
+    two-launch form bit for bit, at odd offsets, small levels (2..5 rows),
+    fixed segment heights and sign-magnitude input with per-band steps; the
+    fused plan has one launch fewer."""
+    from gpu_helpers import make_case
+    w, h, L, x0, y0, seg = shape
+    sm = kind == "signmag"
+    planes = workloads.tile_planes(w, h, L, "97", 3, seed=w + 7 * h, x0=x0, y0=y0,
+                                   ll97=(0.0, 40.0), signmag=sm)
+    steps = [np.linspace(0.5 + 0.1 * c, 1.7, 3 * L + 1).astype(np.float32) for c in range(3)]
+    ck = pkg.COEF_SIGNMAG if sm else pkg.COEF_FLOAT
+    exp_planes, exp = orc.decode_tile(planes, levels=L, filt="97", out_format=2, x0=x0, y0=y0,
+                                      steps=steps, coef_kind=ck)
+    on = pkg.BmiTuning(seg, 0, 0, 0, 0, 1 if seg else 0, 0)
+    off = pkg.BmiTuning(seg, 0, 0, 0, 0, 1 if seg else 0, -1)
+    for fmt, want in ((2, exp), (pkg.FMT_NONE, None)):
+        got = {}
+        for name, t in (("fused", on), ("two launches", off)):
+            got[name] = run_fused(pkg, planes, tuning=t, filt="97", levels=L, out_format=fmt,
+                                  x0=x0, y0=y0, steps=steps, coef_kind=ck)
+        if want is not None:
+            for name in got:
+                assert np.array_equal(got[name][1], want), f"{name}: {diff_report(got[name][1], want)}"
+        else:
+            for name in got:
+                _check_planes(got[name][0], exp_planes, "97", name)
+    d, _keep = make_case(pkg, planes, filt="97", levels=L, out_format=2, x0=x0, y0=y0, steps=steps,
+                         coef_kind=ck)
+    p_on, p_off = pkg.BatchPlan([d], on), pkg.BatchPlan([d], off)
+    assert p_on.num_launches == p_off.num_launches - 1
+    p_on.close()
+    p_off.close()

@@ -78,12 +78,21 @@ def test_noalloc_matches_allocating_entry_point(pkg, workloads, orc):
 
 
 def test_noalloc_rejects_batches_over_capacity(pkg, workloads):
+    """A batch whose plan needs more than the array is refused before any
+    device work.  The tile's buffers are real device memory, so even a
+    wrong acceptance could not make the kernels touch memory outside them:
+    an 8192x8192 9/7 tile in two-launch form (two 4096x4096 float scratch
+    planes, 134 MB)."""
     import torch
     assert torch.cuda.is_available(), "GPU tests need a HIP device"
-    # 5000x5000x3 9/7: two 2500x2500x3 float scratch planes alone are 150 MB
-    d = pkg.tile_desc(coef_ptrs=[1 << 40] * 3, coef_pitch=5000, width=5000, height=5000, levels=5,
-                      method=pkg.W9X7, ncomp=3, out_format=2, out_ptr=1 << 41, out_pitch_px=5000)
+    n = 8192
+    coef = torch.zeros((n, n), dtype=torch.float32, device="cuda")
+    out = torch.zeros((n, n * 3), dtype=torch.uint8, device="cuda")
+    d = pkg.tile_desc(coef_ptrs=[coef.data_ptr()], coef_pitch=n, width=n, height=n, levels=5,
+                      method=pkg.W9X7, ncomp=1, out_format=6, check_out=0, out_ptr=out.data_ptr(),
+                      out_pitch_px=n)
     arr = (pkg.BmiTileDesc * 1)(d)
-    rc = pkg.lib().BMI_decode_tiles_noalloc(arr, 1, None, None)
+    two_launch = pkg.BmiTuning(0, 0, 0, 0, 0, 0, -1)
+    rc = pkg.lib().BMI_decode_tiles_noalloc(arr, 1, ctypes.byref(two_launch), None)
     assert rc == pkg.BMI_E_NOMEM
     torch.cuda.synchronize()
