@@ -1885,7 +1885,7 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
             const long target = (long)P->num_cus * P->tuning.min_waves_per_cu;
             const long simds = 4L * P->num_cus;
             long fixed = 0;
-            bool any97 = false, any = false;
+            bool any97 = false, any = false, any2 = false;
             int max_pairs = 1;  // row pairs of the tallest job: no taller segment helps
             for (int j = step_job0; j < (int)P->jobs.size(); ++j) {
                 const LevelJob &J = P->jobs[j];
@@ -1894,11 +1894,14 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                 } else {
                     any = true;
                     any97 = any97 || J.filt == 97;
+                    any2 = any2 || J.fuse2;
                     max_pairs = std::max(max_pairs, (J.b1 - (J.b0 & ~1) + 1) / 2);
                 }
             }
             if (any && !P->tuning.fixed_seg) {
-                const int halo = any97 ? 8 : 4;
+                // a two-level item also re-synthesises about 8 rows' worth
+                // of the level below at each end of its segment
+                const int halo = any2 ? 16 : (any97 ? 8 : 4);
                 auto items_at = [&](int sp) {
                     long total = fixed;
                     for (int j = step_job0; j < (int)P->jobs.size(); ++j) {
@@ -1910,7 +1913,10 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                     return total;
                 };
                 const int tuned = std::min(P->tuning.seg_rows / 2, max_pairs);
-                const int top = std::min(max_pairs, items_at(tuned) <= 4 * simds ? 2 * tuned : tuned);
+                // two-level steps may always go to twice the tuned height:
+                // their halo costs twice as much (four 4096x2160 images per
+                // plan: 64-row segments 38.8 us per image, 32-row 55.5 us)
+                const int top = std::min(max_pairs, (items_at(tuned) <= 4 * simds || any2) ? 2 * tuned : tuned);
                 int best = 1;
                 long best_cost = -1;
                 for (int sp = top; sp >= 1; --sp) {
