@@ -1013,69 +1013,7 @@ __device__ __attribute__((always_inline)) inline void process_item2(
         rmax = max(rmax, r);
     }
     const int qo0 = (ib0 + rmin) >> 1, qo1 = (ib0 + rmax) >> 1;  // inner pairs to emit
-    const int iqe = qo1 + 2;
-    int q = qo0 - 2;   // next inner pair to consume
-    int have = rmin;   // rows below `have` (from rmin on) are in the ring
-
-    // inner pipeline state (top-down): s0 = H_{q-1}, s1 = e3_{q-1},
-    // s2 = o4_{q-2}, s3 = e5_{q-2}; one pair of loads in flight
-    T t0[NC][2], t1[NC][2], t2[NC][2], t3[NC][2];
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int e = 0; e < 2; ++e) t0[c][e] = t1[c][e] = t2[c][e] = t3[c][e] = 0.0f;
-    uint32_t islot[NC][4];
-    ifetch(q, islot);
-
-    auto inner_step = [&]() {
-        T smp[NC][4];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            smp[c][0] = to_sample<FILT, SK_ILL>(islot[c][0], ill_float, ign[c][0]);
-            smp[c][1] = to_sample<FILT, SK_DET>(islot[c][1], det_float, ign[c][1]);
-            smp[c][2] = to_sample<FILT, SK_DET>(islot[c][2], det_float, ign[c][2]);
-            smp[c][3] = to_sample<FILT, SK_DET>(islot[c][3], det_float, ign[c][3]);
-        }
-        ifetch(min(q + 1, iqe), islot);
-        const int po = q - 2;  // pair emitted by this step
-        const int r0 = 2 * po - ib0;  // its first row, relative
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            T tl = smp[c][0], th = smp[c][1], bl = smp[c][2], bh = smp[c][3];
-            hlift<FILT>(tl, th);
-            hlift<FILT>(bl, bh);
-            const T Lk[2] = {tl, th}, Hk[2] = {bl, bh};
-            T ev[2], od[2];
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                const float e3 = lift97(Lk[e], kDelta, t0[c][e], Hk[e]);
-                const float o4 = lift97(t0[c][e], kGamma, t1[c][e], e3);
-                const float e5 = lift97(t1[c][e], kBeta, t2[c][e], o4);
-                const float o6 = lift97(t2[c][e], kAlpha, t3[c][e], e5);
-                ev[e] = t3[c][e];
-                od[e] = o6;
-                t0[c][e] = Hk[e];
-                t1[c][e] = e3;
-                t2[c][e] = o4;
-                t3[c][e] = e5;
-            }
-            if (po >= qo0) {
-                // rows r0 (even) and r0 + 1 (odd) of the inner output
-                if (r0 >= 0 && r0 < inh) {
-                    float *row = ring + (((r0 & (kRingRows - 1)) * NC + c) << 6);
-                    if (we) row[xe] = ev[0];
-                    if (wo) row[xo] = ev[1];
-                }
-                if (r0 + 1 >= 0 && r0 + 1 < inh) {
-                    float *row = ring + ((((r0 + 1) & (kRingRows - 1)) * NC + c) << 6);
-                    if (we) row[xe] = od[0];
-                    if (wo) row[xo] = od[1];
-                }
-            }
-        }
-        if (po >= qo0) have = max(have, r0 + 2);
-        ++q;
-    };
+    const int iqs = qo0 - 2, iqe = qo1 + 2;  // inner pipeline range, warm-up included
 
     Emitter<FILT, NC, OUTK, PL> em;
     em.init(J);
@@ -1084,80 +1022,177 @@ __device__ __attribute__((always_inline)) inline void process_item2(
     em.set_lanes(ce_ok, co_ok);
     const bool full = em.full_ok();
 
-    // final-level pipeline state (top-down): s0 = H_{k-1}, s1 = e3_{k-1},
-    // s2 = o4_{k-2}, s3 = e5_{k-2}
-    T s0[NC][2], s1[NC][2], s2[NC][2], s3[NC][2];
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int e = 0; e < 2; ++e) s0[c][e] = s1[c][e] = s2[c][e] = s3[c][e] = 0.0f;
-
-    uint32_t slot[D][NC][3];
-#pragma unroll
-    for (int d = 0; d < D; ++d) fetch(min(ks + d, ke), slot[d]);
-    // the LL samples of the next pair, read from the ring one pair ahead
-    // (the inner level runs as far as that pair needs)
-    float llnext[NC];
-    auto read_ll = [&](int k) {
-        const int rk = ll_row(min(k, ke));
-#pragma unroll 1
-        while (have <= rk && q <= iqe) inner_step();
-        const float *row = ring + (((rk & (kRingRows - 1)) * NC) << 6);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) llnext[c] = row[(c << 6) + xl];
+    // One lifting step of a pipeline (the final level's or the inner
+    // level's), as process_item: top-down pair k in, pair k - 2 out;
+    // bottom-up pair k in, pair k + 2 out.
+    // top-down:  p0 = H_{k-1}, p1 = e3_{k-1}, p2 = o4_{k-2}, p3 = e5_{k-2}
+    // bottom-up: p0 = L_{k+1}, p1 = H_{k+1}, p2 = e3_{k+2}, p3 = o4_{k+2},
+    //            p4 = e5_{k+3}
+    auto lift_step = [](auto dir, float Lk, float Hk, float &p0, float &p1, float &p2, float &p3,
+                        float &p4, float &ev, float &od) {
+        constexpr bool REV = decltype(dir)::value;
+        if constexpr (!REV) {
+            const float e3 = lift97(Lk, kDelta, p0, Hk);
+            const float o4 = lift97(p0, kGamma, p1, e3);
+            const float e5 = lift97(p1, kBeta, p2, o4);
+            const float o6 = lift97(p2, kAlpha, p3, e5);
+            ev = p3;
+            od = o6;
+            p0 = Hk;
+            p1 = e3;
+            p2 = o4;
+            p3 = e5;
+        } else {
+            const float e3 = lift97(p0, kDelta, Hk, p1);
+            const float o4 = lift97(p1, kGamma, e3, p2);
+            const float e5 = lift97(p2, kBeta, o4, p3);
+            const float o6 = lift97(p3, kAlpha, e5, p4);
+            ev = e5;
+            od = o6;
+            p0 = Lk;
+            p1 = Hk;
+            p2 = e3;
+            p3 = o4;
+            p4 = e5;
+        }
     };
-    read_ll(ks);
-    const int rounds = (ke - ks + D) / D;
-    for (int it = 0; it < rounds; ++it) {
+
+    // Odd segments run bottom-up, both levels, as in process_item: the rows
+    // two vertically neighbouring segments both read (their halos) are then
+    // read by both at about the same time and come from L2 the second time.
+    auto run = [&](auto dir) {
+        constexpr bool REV = decltype(dir)::value;
+        // inner level state; one pair of loads in flight
+        T t0[NC][2], t1[NC][2], t2[NC][2], t3[NC][2], t4[NC][2];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const int k = ks + it * D + d;
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) t0[c][e] = t1[c][e] = t2[c][e] = t3[c][e] = t4[c][e] = 0.0f;
+        int q = REV ? iqe : iqs;  // next inner pair to consume
+        int have = rmin;          // top-down: rows [rmin, have) are in the ring
+        int low = rmax + 1;       // bottom-up: rows [low, rmax] are in the ring
+        uint32_t islot[NC][4];
+        ifetch(q, islot);
+
+        auto inner_step = [&]() {
             T smp[NC][4];
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                smp[c][0] = __fmul_rn(llnext[c], gn[c][0]);
-                smp[c][1] = to_sample<FILT, SK_DET>(slot[d][c][0], det_float, gn[c][1]);
-                smp[c][2] = to_sample<FILT, SK_DET>(slot[d][c][1], det_float, gn[c][2]);
-                smp[c][3] = to_sample<FILT, SK_DET>(slot[d][c][2], det_float, gn[c][3]);
+                smp[c][0] = to_sample<FILT, SK_ILL>(islot[c][0], ill_float, ign[c][0]);
+                smp[c][1] = to_sample<FILT, SK_DET>(islot[c][1], det_float, ign[c][1]);
+                smp[c][2] = to_sample<FILT, SK_DET>(islot[c][2], det_float, ign[c][2]);
+                smp[c][3] = to_sample<FILT, SK_DET>(islot[c][3], det_float, ign[c][3]);
             }
-            fetch(min(k + D, ke), slot[d]);
-            read_ll(k + 1);
-
-            T evn[2][NC], odd[2][NC];
+            ifetch(REV ? max(q - 1, iqs) : min(q + 1, iqe), islot);
+            const int po = REV ? q + 2 : q - 2;  // pair emitted by this step
+            const int r0 = 2 * po - ib0;         // its first row, relative
+            const bool emit = po >= qo0 && po <= qo1;
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
                 T tl = smp[c][0], th = smp[c][1], bl = smp[c][2], bh = smp[c][3];
                 hlift<FILT>(tl, th);
                 hlift<FILT>(bl, bh);
                 const T Lk[2] = {tl, th}, Hk[2] = {bl, bh};
+                T ev[2], od[2];
 #pragma unroll
-                for (int e = 0; e < 2; ++e) {
-                    const float e3 = lift97(Lk[e], kDelta, s0[c][e], Hk[e]);
-                    const float o4 = lift97(s0[c][e], kGamma, s1[c][e], e3);
-                    const float e5 = lift97(s1[c][e], kBeta, s2[c][e], o4);
-                    const float o6 = lift97(s2[c][e], kAlpha, s3[c][e], e5);
-                    evn[e][c] = s3[c][e];
-                    odd[e][c] = o6;
-                    s0[c][e] = Hk[e];
-                    s1[c][e] = e3;
-                    s2[c][e] = o4;
-                    s3[c][e] = e5;
+                for (int e = 0; e < 2; ++e)
+                    lift_step(dir, Lk[e], Hk[e], t0[c][e], t1[c][e], t2[c][e], t3[c][e], t4[c][e],
+                              ev[e], od[e]);
+                if (emit) {
+                    // rows r0 (even) and r0 + 1 (odd) of the inner output
+                    if (r0 >= 0 && r0 < inh) {
+                        float *row = ring + (((r0 & (kRingRows - 1)) * NC + c) << 6);
+                        if (we) row[xe] = ev[0];
+                        if (wo) row[xo] = ev[1];
+                    }
+                    if (r0 + 1 >= 0 && r0 + 1 < inh) {
+                        float *row = ring + ((((r0 + 1) & (kRingRows - 1)) * NC + c) << 6);
+                        if (we) row[xe] = od[0];
+                        if (wo) row[xo] = od[1];
+                    }
                 }
             }
-            const int ko = k - 2;
-            if (ko >= kout0 && ko <= kout1) {
-                if (full && 2 * ko >= b0 && 2 * ko + 1 < b1) {
-                    em.put2_full(2 * ko, ce, evn[0], evn[1], ce_ok);
-                    em.put2_full(2 * ko + 1, ce, odd[0], odd[1], ce_ok);
-                } else {
-                    if (2 * ko >= b0 && 2 * ko < b1) em.put2(2 * ko, ce, evn[0], evn[1], ce_ok, co_ok);
-                    if (2 * ko + 1 >= b0 && 2 * ko + 1 < b1)
-                        em.put2(2 * ko + 1, ce, odd[0], odd[1], ce_ok, co_ok);
-                }
+            if (emit) {
+                have = max(have, r0 + 2);
+                low = min(low, r0);
             }
-            __builtin_amdgcn_sched_barrier(0);
+            q += REV ? -1 : 1;
+        };
+
+        // final-level pipeline state
+        T s0[NC][2], s1[NC][2], s2[NC][2], s3[NC][2], s4[NC][2];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) s0[c][e] = s1[c][e] = s2[c][e] = s3[c][e] = s4[c][e] = 0.0f;
+
+        uint32_t slot[D][NC][3];
+#pragma unroll
+        for (int d = 0; d < D; ++d) fetch(REV ? max(ke - d, ks) : min(ks + d, ke), slot[d]);
+        // the LL samples of the next pair, read from the ring one pair ahead
+        // (the inner level runs as far as that pair needs)
+        float llnext[NC];
+        auto read_ll = [&](int k) {
+            const int rk = ll_row(min(max(k, ks), ke));
+            if constexpr (REV) {
+#pragma unroll 1
+                while (low > rk && q >= iqs) inner_step();
+            } else {
+#pragma unroll 1
+                while (have <= rk && q <= iqe) inner_step();
+            }
+            const float *row = ring + (((rk & (kRingRows - 1)) * NC) << 6);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) llnext[c] = row[(c << 6) + xl];
+        };
+        read_ll(REV ? ke : ks);
+        const int rounds = (ke - ks + D) / D;
+        for (int it = 0; it < rounds; ++it) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int k = REV ? ke - (it * D + d) : ks + it * D + d;
+                T smp[NC][4];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    smp[c][0] = __fmul_rn(llnext[c], gn[c][0]);
+                    smp[c][1] = to_sample<FILT, SK_DET>(slot[d][c][0], det_float, gn[c][1]);
+                    smp[c][2] = to_sample<FILT, SK_DET>(slot[d][c][1], det_float, gn[c][2]);
+                    smp[c][3] = to_sample<FILT, SK_DET>(slot[d][c][2], det_float, gn[c][3]);
+                }
+                fetch(REV ? max(k - D, ks) : min(k + D, ke), slot[d]);
+                read_ll(REV ? k - 1 : k + 1);
+
+                T evn[2][NC], odd[2][NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    T tl = smp[c][0], th = smp[c][1], bl = smp[c][2], bh = smp[c][3];
+                    hlift<FILT>(tl, th);
+                    hlift<FILT>(bl, bh);
+                    const T Lk[2] = {tl, th}, Hk[2] = {bl, bh};
+#pragma unroll
+                    for (int e = 0; e < 2; ++e)
+                        lift_step(dir, Lk[e], Hk[e], s0[c][e], s1[c][e], s2[c][e], s3[c][e], s4[c][e],
+                                  evn[e][c], odd[e][c]);
+                }
+                const int ko = REV ? k + 2 : k - 2;
+                if (ko >= kout0 && ko <= kout1) {
+                    if (full && 2 * ko >= b0 && 2 * ko + 1 < b1) {
+                        em.put2_full(2 * ko, ce, evn[0], evn[1], ce_ok);
+                        em.put2_full(2 * ko + 1, ce, odd[0], odd[1], ce_ok);
+                    } else {
+                        if (2 * ko >= b0 && 2 * ko < b1) em.put2(2 * ko, ce, evn[0], evn[1], ce_ok, co_ok);
+                        if (2 * ko + 1 >= b0 && 2 * ko + 1 < b1)
+                            em.put2(2 * ko + 1, ce, odd[0], odd[1], ce_ok, co_ok);
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
-    }
+    };
+    if (seg & 1)
+        run(std::true_type{});
+    else
+        run(std::false_type{});
 }
 
 // Kernel variants: V = (filt == 97) * 32 + sub * 4 + kd.
