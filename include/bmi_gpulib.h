@@ -154,6 +154,13 @@ typedef struct BmiTileDesc {
     int out_x, out_y;                /* tile position in `out`, pixels */
     void *planes_out[MAX_COMPONENT]; /* NONE: per-component result planes */
     int planes_pitch;                /* elements */
+    unsigned int ignore_mask;        /* bit c set: component c is not decoded
+                                        (the reference's mIgnoreComponents,
+                                        decoder/common/include/codec_base.h:215-228);
+                                        the others fill the output slots in
+                                        order (bmi_cpu_dwt_decoder.cpp:1180-1290),
+                                        an ignored check-out component means
+                                        none; RGB formats need three left */
 } BmiTileDesc;
 
 /* Launch shape of the fused kernel (0 = library default / autotuned). */

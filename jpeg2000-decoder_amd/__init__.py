@@ -83,6 +83,7 @@ class BmiTileDesc(ctypes.Structure):
         ("out_y", ctypes.c_int),
         ("planes_out", ctypes.c_void_p * MAX_COMPONENT),
         ("planes_pitch", ctypes.c_int),
+        ("ignore_mask", ctypes.c_uint),
     ]
 
 
@@ -416,7 +417,7 @@ def check(rc: int, what: str) -> None:
 def tile_desc(*, coef_ptrs, coef_pitch, width, height, x0=0, y0=0, levels, method,
               ncomp, coef_kind=COEF_FLOAT, bit_depth=8, out_format=FMT_NONE, check_out=-1,
               mct=None, r53_mode=R53_REFERENCE, steps=None, out_ptr=0, out_pitch_px=0,
-              out_x=0, out_y=0, planes_ptrs=None, planes_pitch=0, top_level=-1) -> BmiTileDesc:
+              out_x=0, out_y=0, planes_ptrs=None, planes_pitch=0, top_level=-1, ignore_mask=0) -> BmiTileDesc:
     """Fill a BmiTileDesc; steps is [ncomp][3L+1] (defaults to 1.0)."""
     d = BmiTileDesc()
     for c, p in enumerate(coef_ptrs):
@@ -439,6 +440,7 @@ def tile_desc(*, coef_ptrs, coef_pitch, width, height, x0=0, y0=0, levels, metho
         for c, p in enumerate(planes_ptrs):
             d.planes_out[c] = p
     d.planes_pitch = planes_pitch
+    d.ignore_mask = ignore_mask
     return d
 
 
@@ -733,11 +735,16 @@ class CodestreamDecoder:
     (e.g. timed) on the resident inputs."""
 
     def __init__(self, data: bytes, *, flags: int = CS_REFERENCE, out_format=None, check_out=None,
-                 r53_mode: int = R53_REFERENCE, top_level: int = -1, device: str = "cuda"):
+                 r53_mode: int = R53_REFERENCE, top_level: int = -1, ignore_mask: int = 0,
+                 device: str = "cuda"):
         """top_level: synthesis levels to run (-1 all; k < levels gives the
         image at 2^-(levels-k) of full size, 0 the LL thumbnail, the
         reference's mDwtDecodeTopLevel / GetThumbnail).  Tier 1 then decodes
-        only the code-blocks of the bands that resolution needs."""
+        only the code-blocks of the bands that resolution needs.
+        ignore_mask: components not to decode (bit c = component c, the
+        reference's mIgnoreComponents); their code-blocks are not decoded
+        either, and the remaining components fill the output slots in
+        order."""
         import numpy as np
         import torch
         self.cs = cs = Codestream(data, flags)
@@ -776,6 +783,7 @@ class CodestreamDecoder:
             desc.check_out = self.check_out
             desc.r53_mode = r53_mode
             desc.top_level = top_level if reduce else -1
+            desc.ignore_mask = ignore_mask
             ox, oy = cd(x0) - cd(info.x0), cd(y0) - cd(info.y0)
             if self.out is not None:
                 desc.out = self.out.data_ptr()
@@ -788,16 +796,21 @@ class CodestreamDecoder:
             descs.append(desc)
             _check_block_table(blocks, nb, ptrs, pitch, h, seg_base, sb, segs)
             size = ctypes.sizeof(BmiT1Block)
-            if reduce and desc.levels > 0:
-                # keep the blocks inside the Mallat region of the target
-                # resolution: the LL band and the detail bands of the levels
-                # that are synthesised
-                rw, rh = cd(x0 + w) - cd(x0), cd(y0 + h) - cd(y0)
+            if (reduce and desc.levels > 0) or ignore_mask:
+                # keep the blocks of the components decoded, inside the
+                # Mallat region of the target resolution: the LL band and
+                # the detail bands of the levels that are synthesised
+                if reduce and desc.levels > 0:
+                    rw, rh = cd(x0 + w) - cd(x0), cd(y0 + h) - cd(y0)
+                else:
+                    rw, rh = w, h
                 raw = bytes(blocks)
                 keep = []
                 for i in range(nb):
                     b = blocks[i]
                     c = next(k for k, p in enumerate(ptrs) if p <= b.out < p + 4 * pitch * h)
+                    if (ignore_mask >> c) & 1:
+                        continue
                     y, x = divmod((b.out - ptrs[c]) // 4, pitch)
                     if x < rw and y < rh:
                         keep.append(raw[i * size:(i + 1) * size])

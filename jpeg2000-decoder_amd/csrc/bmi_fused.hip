@@ -1555,6 +1555,39 @@ void needed_comps(const BmiTileDesc &t, int ok, int *list, int *n)
 
 }  // namespace
 
+int bmi_apply_ignore(const BmiTileDesc &in, BmiTileDesc *out)
+{
+    BmiTileDesc t = in;
+    const unsigned mask = in.ignore_mask & ((1u << MAX_COMPONENT) - 1);
+    if (in.ncomp < 1 || in.ncomp > MAX_COMPONENT) return BMI_E_INVALID;
+    if (mask) {
+        int n = 0, co = -1;
+        for (int c = 0; c < in.ncomp; ++c) {
+            if ((mask >> c) & 1) continue;
+            t.coef[n] = in.coef[c];
+            t.planes_out[n] = in.planes_out[c];
+            memcpy(t.steps[n], in.steps[c], sizeof(t.steps[n]));
+            if (in.check_out == c) co = n;
+            ++n;
+        }
+        if (n == 0) return BMI_E_INVALID;
+        for (int c = n; c < MAX_COMPONENT; ++c) {
+            t.coef[c] = nullptr;
+            t.planes_out[c] = nullptr;
+        }
+        t.ncomp = n;
+        // an ignored (or missing) check-out component means no check-out
+        // (bmi_cpu_dwt_decoder.cpp:1222-1225); RGB check-outs name output
+        // channels, which keep their meaning
+        const int ok = out_kind_for(in);
+        if (ok != OUT_RGB) t.check_out = in.check_out >= 0 ? co : -1;
+        if (ok == OUT_RGB && n < 3) return BMI_E_UNSUPPORTED;
+    }
+    t.ignore_mask = 0;
+    *out = t;
+    return BMI_OK;
+}
+
 // A tile runs its last two synthesis levels as one two-level step when
 // both levels are 9/7, at least two samples in each direction, and a
 // two-level kernel exists for its final level's variant.
@@ -1599,10 +1632,15 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
 {
     if (!out_plan || !tiles || ntiles < 1) return bmi_fail(BMI_E_INVALID, "batch_plan_create: bad args");
     *out_plan = nullptr;
+    // ignored components are dropped here: everything below sees the tile
+    // with its remaining components in the lower slots
+    std::vector<BmiTileDesc> eff(tiles, tiles + ntiles);
     for (int i = 0; i < ntiles; ++i) {
-        int rc = validate_tile(tiles[i]);
+        int rc = bmi_apply_ignore(tiles[i], &eff[i]);
+        if (!rc) rc = validate_tile(eff[i]);
         if (rc) return bmi_fail(rc, "batch_plan_create: tile %d rejected", i);
     }
+    tiles = eff.data();
     BmiBatchPlan *P = new BmiBatchPlan();
     P->tiles.assign(tiles, tiles + ntiles);
     P->tuning.seg_rows = default_seg_rows(tuning);
@@ -2064,10 +2102,13 @@ size_t bmi_plan_table_bytes(const BmiBatchPlan *P)
 // and upload it on `stream` (asynchronous; the plan's host vectors stay
 // alive with the plan).  The counters are cleared with a memset, so a
 // caller that captures graphs must bind outside the capture.
-size_t bmi_plan_estimate_bytes(const BmiTileDesc *t, const BmiTuning *tuning)
+size_t bmi_plan_estimate_bytes(const BmiTileDesc *tin, const BmiTuning *tuning)
 {
     (void)tuning;
-    if (!t || validate_tile(*t) != BMI_OK) return 0;
+    BmiTileDesc eff;
+    if (!tin || bmi_apply_ignore(*tin, &eff) != BMI_OK) return 0;
+    const BmiTileDesc *t = &eff;
+    if (validate_tile(*t) != BMI_OK) return 0;
     const int L = t->levels;
     const int Ls = (t->top_level < 0) ? L : t->top_level;
     int maxw = 0, maxh = 0;
@@ -2333,10 +2374,13 @@ extern "C" int BMI_batch_plan_execute_graph(BmiBatchPlan *P, StreamId stream)
     return BMI_OK;
 }
 
-extern "C" int BMI_batch_plan_rebind(BmiBatchPlan *P, int i, const BmiTileDesc *t)
+extern "C" int BMI_batch_plan_rebind(BmiBatchPlan *P, int i, const BmiTileDesc *tin)
 {
-    if (!P || !t || i < 0 || i >= (int)P->tiles.size())
+    if (!P || !tin || i < 0 || i >= (int)P->tiles.size())
         return bmi_fail(BMI_E_INVALID, "batch_plan_rebind: bad args");
+    BmiTileDesc eff;
+    if (int rc0 = bmi_apply_ignore(*tin, &eff)) return bmi_fail(rc0, "batch_plan_rebind: tile rejected");
+    const BmiTileDesc *t = &eff;
     const BmiTileDesc &o = P->tiles[i];
     if (t->width != o.width || t->height != o.height || t->levels != o.levels ||
         t->ncomp != o.ncomp || t->method != o.method || t->out_format != o.out_format ||

@@ -37,3 +37,12 @@ size_t bmi_plan_table_bytes(const BmiBatchPlan *plan);
 size_t bmi_plan_estimate_bytes(const BmiTileDesc *tile, const BmiTuning *tuning);
 int bmi_plan_bind(BmiBatchPlan *plan, void *base, hipStream_t stream, void *staging,
                   size_t staging_bytes);
+
+// The reference's ignore-component mask (mIgnoreComponents): the tile with
+// its ignored components removed and the rest moved down to the lower
+// component slots, in order (coefficient planes, steps, result planes), the
+// check-out component renumbered (-1 when it is ignored).  Returns BMI_OK,
+// BMI_E_INVALID when every component is ignored, or BMI_E_UNSUPPORTED for
+// an RGB format with fewer than three components left (the reference's
+// MergeComponents asserts on RGB formats).  `out` may alias `in`.
+int bmi_apply_ignore(const BmiTileDesc &in, BmiTileDesc *out);

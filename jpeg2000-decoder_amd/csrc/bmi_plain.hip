@@ -680,9 +680,13 @@ extern "C" void BMI_tile_One_Component(TileInfo_c *tile, int component_id, Encod
 
 // The CudaManager sequence for one tile (cudainterface.cpp:1241-1630):
 // BMI_comp_idwt per component, then ICT / MCT / Merge into the result.
-extern "C" int BMI_plain_decode(const BmiTileDesc *t, void *temp, size_t temp_bytes, StreamId stream)
+extern "C" int BMI_plain_decode(const BmiTileDesc *tin, void *temp, size_t temp_bytes, StreamId stream)
 {
-    if (!t || !temp) return bmi_fail(BMI_E_INVALID, "plain_decode: null argument");
+    if (!tin || !temp) return bmi_fail(BMI_E_INVALID, "plain_decode: null argument");
+    // ignored components are not decoded; the rest fill the lower slots
+    BmiTileDesc eff;
+    if (int rc = bmi_apply_ignore(*tin, &eff)) return bmi_fail(rc, "plain_decode: ignore mask");
+    const BmiTileDesc *t = &eff;
     if (t->levels < 0 || t->levels > MAX_DWT_LEVEL || t->ncomp < 1 || t->ncomp > 4 ||
         t->width < 1 || t->height < 1)
         return bmi_fail(BMI_E_INVALID, "plain_decode: bad tile");

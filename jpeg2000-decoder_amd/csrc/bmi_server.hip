@@ -375,6 +375,12 @@ extern "C" int BMI_server_create(BmiServer **out, const BmiServerConfig *cfg)
 extern "C" int BMI_server_submit(BmiServer *S, const BmiRequest *req, unsigned long long *ticket)
 {
     if (!S || !req || !ticket) return bmi_fail(BMI_E_INVALID, "server_submit: bad args");
+    // ignored components are dropped at submit: the request is queued with
+    // the remaining ones in the lower slots
+    BmiRequest eff = *req;
+    if (int rc = bmi_apply_ignore(req->tile, &eff.tile))
+        return bmi_fail(rc, "server_submit: ignore mask");
+    req = &eff;
     const BmiTileDesc &t = req->tile;
     if (t.out_format == IMAGE_FOTMAT_NONE || pixel_bytes(t.out_format) == 0)
         return bmi_fail(BMI_E_UNSUPPORTED, "server_submit: output format %d not served",

@@ -196,12 +196,33 @@ def one_component(words: np.ndarray, mode: int, comp_id: int, bit_depth: int) ->
     return out
 
 
+def ignore_components(planes, steps, check_out: int, out_format: int, ignore_mask: int):
+    """mIgnoreComponents (codec_base.h:215-228): the components whose bit is
+    set are not decoded (InitPicture skips them, cudainterface.cpp:545-560,
+    648-653); the remaining ones fill the component slots in order
+    (bmi_cpu_dwt_decoder.cpp:1180-1290), and a check-out component that is
+    ignored means no check-out (:1222-1225).  Returns (planes, steps,
+    check_out) of the remaining components; RGB formats keep their output
+    channel check-out."""
+    keep = [c for c in range(planes.shape[0]) if not (ignore_mask >> c) & 1]
+    if not keep:
+        raise ValueError("every component ignored")
+    rgb = out_format in (0, 1, 2, 8)
+    if not rgb and check_out >= 0:
+        check_out = keep.index(check_out) if check_out in keep else -1
+    st = None if steps is None else [steps[c] for c in keep]
+    return planes[keep], st, check_out
+
+
 def decode_tile(planes: np.ndarray, *, levels: int, filt: str, out_format: int, bit_depth: int = 8,
                 check_out: int = -1, x0=0, y0=0, toplevel=-1, steps=None,
-                coef_kind=COEF_FLOAT, r53_mode=R53_REFERENCE):
+                coef_kind=COEF_FLOAT, r53_mode=R53_REFERENCE, ignore_mask: int = 0):
     """Reference CPU pipeline for one tile: IDWT per component, then
     RCT/ICT (RGB formats), merge (*_COMPONENTS) or the planes (NONE).
     Returns (recon planes list, packed output or None)."""
+    if ignore_mask:
+        planes, steps, check_out = ignore_components(planes, steps, check_out, out_format,
+                                                     ignore_mask)
     ncomp = planes.shape[0]
     rgb = out_format in (0, 1, 2, 8)
     recon = []

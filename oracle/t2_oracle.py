@@ -676,7 +676,7 @@ def tile_planes(tile, orc):
 
 
 def decode_image(buf: bytes, orc, *, out_format: int, check_out: int = -1, r53_mode: int = 0,
-                 guard_t800: bool = False, toplevel: int = -1):
+                 guard_t800: bool = False, toplevel: int = -1, ignore_mask: int = 0):
     """Whole-image oracle decode: T2 (this module), T1 and the reference
     pipeline (oracle.decode_tile) per tile, tiles placed in one packed
     image (rows of width * pixel bytes), or per-component planes for
@@ -702,12 +702,13 @@ def decode_image(buf: bytes, orc, *, out_format: int, check_out: int = -1, r53_m
                                       out_format=out_format, bit_depth=depth, check_out=check_out,
                                       x0=T["tx0"], y0=T["ty0"], steps=steps,
                                       coef_kind=orc.COEF_SIGNMAG, r53_mode=r53_mode,
-                                      toplevel=toplevel if sh else -1)
+                                      toplevel=toplevel if sh else -1, ignore_mask=ignore_mask)
         ox, oy = cd(T["tx0"]) - cd(siz["XOsiz"]), cd(T["ty0"]) - cd(siz["YOsiz"])
         h, w = cd(T["ty1"]) - cd(T["ty0"]), cd(T["tx1"]) - cd(T["tx0"])
         if out_format == 7:
-            for c in range(nc):
-                out[c][oy:oy + h, ox:ox + w] = np.asarray(rec[c]).view(np.int32)
+            kept = [c for c in range(nc) if not (ignore_mask >> c) & 1]
+            for i, c in enumerate(kept):
+                out[c][oy:oy + h, ox:ox + w] = np.asarray(rec[i]).view(np.int32)
         else:
             out[oy:oy + h, ox * pb:(ox + w) * pb] = packed
     return out

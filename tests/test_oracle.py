@@ -398,3 +398,33 @@ def test_cpu_baseline_equals_oracle(orc, workloads, threads, shape):
     for c in range(3):
         assert np.array_equal(got_planes[c].view(np.uint32), exp_planes[c].view(np.uint32)), c
     assert np.array_equal(got, exp)
+
+
+def test_ignore_components_slot_order(orc):
+    """mIgnoreComponents: the remaining components move down to the lower
+    output slots in order (bmi_cpu_dwt_decoder.cpp:1180-1290); a four-
+    component 8-bit merge with component 1 ignored packs components 0, 2, 3
+    into a three-component pixel exactly as a three-component tile made of
+    them; an ignored check-out component means none; nothing changes for a
+    mask of 0."""
+    rng = np.random.default_rng(12)
+    planes = rng.integers(-50, 50, (4, 6, 7)).astype(np.int32)
+    _, a = orc.decode_tile(planes, levels=0, filt="53", out_format=6, ignore_mask=0b0010)
+    _, b = orc.decode_tile(planes[[0, 2, 3]], levels=0, filt="53", out_format=6)
+    assert np.array_equal(a, b)
+    # byte slots of the 24-bit merge: comp2 | comp1 | comp0 of the remaining
+    # list, i.e. planes 3, 2, 0 (hand check of one pixel)
+    px = a.reshape(6, 7, 3)[0, 0]
+    assert px.tolist() == [planes[3, 0, 0] + 128, planes[2, 0, 0] + 128, planes[0, 0, 0] + 128]
+    # check-out of an ignored component: no check-out
+    _, c = orc.decode_tile(planes, levels=0, filt="53", out_format=6, check_out=1, ignore_mask=0b0010)
+    assert np.array_equal(c, a)
+    # check-out of a kept one is renumbered: plane 2 is slot 1
+    _, d = orc.decode_tile(planes, levels=0, filt="53", out_format=6, check_out=2, ignore_mask=0b0010)
+    _, e = orc.decode_tile(planes[[0, 2, 3]], levels=0, filt="53", out_format=6, check_out=1)
+    assert np.array_equal(d, e)
+    _, f = orc.decode_tile(planes, levels=0, filt="53", out_format=3)
+    _, g = orc.decode_tile(planes, levels=0, filt="53", out_format=3, ignore_mask=0)
+    assert np.array_equal(f, g)
+    with pytest.raises(ValueError):
+        orc.decode_tile(planes, levels=0, filt="53", out_format=3, ignore_mask=0b1111)
