@@ -112,35 +112,43 @@ def _device_planes(torch, dev, w, h, filt, ncomp, gen):
             for _ in range(ncomp)]
 
 
-def batched_leg(pkg, wl, torch, dev, stream, steps, warmup):
+def batched_leg(pkg, wl, torch, dev, stream, steps, warmup, inflight=2):
     """256 tiles (sides 64..1024, 3..6 levels, 5/3 / 9/7, 1 or 3
     components) in one plan: autotuned launch shape, hipGraph replay and
-    direct launches timed (the faster reported);
-    beside it the plain path decoding the same tiles one per call."""
+    direct launches timed.  Reported: batches decoded as a stream,
+    `inflight` plans (each with its own buffer set) on their own streams;
+    one batch at a time beside it, and the plain path decoding the same
+    tiles one per call."""
     tiles = wl.config3_params(0)
     gen = torch.Generator(device=dev)
     gen.manual_seed(0)
-    descs, keep = [], []
+    sets = []
     pixels = comp_samples = out_bytes = 0
-    for t in tiles:
-        coef = _device_planes(torch, dev, t["w"], t["h"], t["filt"], t["ncomp"], gen)
-        fmt, co = (2, -1) if t["ncomp"] == 3 else (6, 0)
-        out = torch.zeros((t["h"], t["w"] * 3), dtype=torch.uint8, device=dev)
-        descs.append(pkg.tile_desc(coef_ptrs=[c.data_ptr() for c in coef], coef_pitch=t["w"],
-                                   width=t["w"], height=t["h"], levels=t["levels"],
-                                   method=pkg.W5X3 if t["filt"] == "53" else pkg.W9X7,
-                                   ncomp=t["ncomp"], bit_depth=8, out_format=fmt, check_out=co,
-                                   out_ptr=out.data_ptr(), out_pitch_px=t["w"]))
-        keep.append((coef, out))
-        pixels += t["w"] * t["h"]
-        comp_samples += t["w"] * t["h"] * t["ncomp"]
-        out_bytes += t["w"] * t["h"] * 3
+    for k in range(max(1, inflight)):
+        descs, keep = [], []
+        for t in tiles:
+            coef = _device_planes(torch, dev, t["w"], t["h"], t["filt"], t["ncomp"], gen)
+            fmt, co = (2, -1) if t["ncomp"] == 3 else (6, 0)
+            out = torch.zeros((t["h"], t["w"] * 3), dtype=torch.uint8, device=dev)
+            descs.append(pkg.tile_desc(coef_ptrs=[c.data_ptr() for c in coef], coef_pitch=t["w"],
+                                       width=t["w"], height=t["h"], levels=t["levels"],
+                                       method=pkg.W5X3 if t["filt"] == "53" else pkg.W9X7,
+                                       ncomp=t["ncomp"], bit_depth=8, out_format=fmt, check_out=co,
+                                       out_ptr=out.data_ptr(), out_pitch_px=t["w"]))
+            keep.append((coef, out))
+            if k == 0:
+                pixels += t["w"] * t["h"]
+                comp_samples += t["w"] * t["h"] * t["ncomp"]
+                out_bytes += t["w"] * t["h"] * 3
+        sets.append((descs, keep))
+    descs, keep = sets[0]
     sh = stream.cuda_stream
     t0 = time.perf_counter()
     best, best_us = pkg.autotune(descs, None, reps=3, stream=sh)
     tune_s = time.perf_counter() - t0
-    plan = pkg.BatchPlan(descs, best)
-    mode_ms = {}
+    plans = [pkg.BatchPlan(d, best) for d, _ in sets]
+    plan = plans[0]
+    one_ms = {}
     for name, fn in (("hipGraph", lambda: plan.execute_graph(sh)),
                      ("direct launches", lambda: plan.execute(sh))):
         for _ in range(warmup):
@@ -152,9 +160,29 @@ def batched_leg(pkg, wl, torch, dev, stream, steps, warmup):
             fn()
         e1.record(stream)
         torch.cuda.synchronize()
+        one_ms[name] = e0.elapsed_time(e1) / steps
+    # a stream of batches: plan i % inflight on stream i % inflight
+    streams = [torch.cuda.Stream(dev) for _ in plans]
+    mode_ms = {}
+    for name, graph in (("hipGraph", True), ("direct launches", False)):
+        for rep in range(2):
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for st_ in streams:
+                st_.wait_stream(stream)
+            n = steps if rep else max(warmup, len(plans))
+            for i in range(n):
+                p_, ss = plans[i % len(plans)], streams[i % len(plans)].cuda_stream
+                p_.execute_graph(ss) if graph else p_.execute(ss)
+            for st_ in streams:
+                stream.wait_stream(st_)
+            e1.record(stream)
+            torch.cuda.synchronize()
         mode_ms[name] = e0.elapsed_time(e1) / steps
     launch_mode = min(mode_ms, key=mode_ms.get)
     ms = mode_ms[launch_mode]
+    ms_one = min(one_ms.values())
     compulsory = comp_samples * 4 + out_bytes
     res = {"tiles": len(tiles), "pixels": pixels, "ms_per_batch": round(ms, 4),
            "Mpixel_s": round(pixels / (ms * 1e-3) / 1e6, 1),
@@ -162,12 +190,18 @@ def batched_leg(pkg, wl, torch, dev, stream, steps, warmup):
            "GBs_compulsory": round(compulsory / (ms * 1e-3) / 1e9, 1),
            "frac_of_peak": round(compulsory / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "work_items": plan.num_items, "launches": plan.num_launches,
-           "launch_mode": launch_mode,
+           "launch_mode": launch_mode, "batches_in_flight": len(plans),
            "us_per_batch_by_launch_mode": {k: round(v * 1e3, 1) for k, v in mode_ms.items()},
+           "ms_per_batch_one_at_a_time": round(ms_one, 4),
+           "frac_of_peak_one_at_a_time": round(compulsory / (ms_one * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "us_one_at_a_time_by_launch_mode": {k: round(v * 1e3, 1) for k, v in one_ms.items()},
            "tuning": {"seg_rows": best.seg_rows, "waves_per_group": best.waves_per_group,
-                      "min_waves_per_cu": best.min_waves_per_cu, "persistent": best.persistent},
+                      "min_waves_per_cu": best.min_waves_per_cu, "persistent": best.persistent,
+                      "fuse2": best.fuse2},
            "autotune_s": round(tune_s, 2)}
-    plan.close()
+    for p_ in plans:
+        p_.close()
+    del sets[1:]
     # plain path: the drop-in sequence, one tile per call, on copies (in place)
     work = [[torch.empty_like(c) for c in coef] for coef, _ in keep]
     temps = torch.empty(1024 * 1024 + 64, dtype=torch.int32, device=dev)
@@ -194,6 +228,7 @@ def batched_leg(pkg, wl, torch, dev, stream, steps, warmup):
     res["plain_ms_per_batch"] = round(pms, 3)
     res["plain_Mpixel_s"] = round(pixels / (pms * 1e-3) / 1e6, 1)
     res["speedup_vs_plain"] = round(pms / ms, 2)
+    res["speedup_vs_plain_one_at_a_time"] = round(pms / ms_one, 2)
     return res
 
 

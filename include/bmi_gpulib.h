@@ -226,7 +226,8 @@ int BMI_batch_plan_error_flag(BmiBatchPlan *plan);
    mean.  Returns the best in *best (its mean time per execute in
    *best_us).  candidates == NULL selects the built-in grid (segment rows
    16..128 x items per CU 4..16 x 2, 4 or per-launch waves per workgroup x merged or
-   per-variant launches of mixed steps, modes 0 and 3).  Runs the real
+   per-variant launches of mixed steps, modes 0 and 3 x two-level final steps on
+   or off).  Runs the real
    decode: outputs of the tiles are written. */
 int BMI_autotune(const BmiTileDesc *tiles, int ntiles, const BmiTuning *candidates, int ncand,
                  int reps, StreamId stream, BmiTuning *best, float *best_us);

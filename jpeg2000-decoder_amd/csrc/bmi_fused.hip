@@ -2453,11 +2453,13 @@ extern "C" int BMI_autotune(const BmiTileDesc *tiles, int ntiles, const BmiTunin
         cand.assign(candidates, candidates + ncand);
     } else {
         // default grid: segment height x items per CU x waves per group x
-        // (merged or per-variant launches of steps that mix variants)
+        // (merged or per-variant launches of steps that mix variants) x
+        // (the last two 9/7 levels of a tile in one pass or two)
         for (int seg : {16, 32, 64, 128})
             for (int mw : {4, 8, 16})
                 for (int wpg : {0, 2, 4})   // 0: chosen per launch
-                    for (int mode : {0, 3}) cand.push_back(BmiTuning{seg, wpg, 0, mode, mw});
+                    for (int mode : {0, 3})
+                        for (int fuse : {0, -1}) cand.push_back(BmiTuning{seg, wpg, 0, mode, mw, 0, fuse});
     }
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t e0, e1;
