@@ -32,6 +32,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "bmi_device.h"
 #include "bmi_internal.h"
@@ -711,6 +712,621 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup)
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Code-block style 0 (and predictable termination, which changes nothing
+// for the decoder): the blocks the reference decodes, in a kernel of their
+// own that keeps the scalar work per decision small.
+//
+// State.  Per stripe and column one 32-bit flag word, in LDS between passes
+// (4 KB per wave):
+//   bits 3r, 3r+1, 3r+2 (row r = 0..3): significant, negative, coded in
+//     this bit-plane's significance pass ("visited");
+//   bits 12+r: refinable (significant before this bit-plane);
+//   bits 16+r: refinable before the previous bit-plane as well, i.e. not
+//     the first refinement (T.800 Table D.4), which the reference decides
+//     from the magnitude above the plane being below 4: a sample found
+//     significant at plane p has magnitude 3 << (p-1), so the two rules
+//     agree.
+// At the start of each stripe of each pass the lanes build, with vector
+// instructions, every column's 3x3 windows of significance and of signs
+// over rows -1..4 of the stripe: bit 3j + dx + 1 holds row j-1, column
+// x+dx.  Row r's neighbourhood is then the 9-bit field at bit 3r
+// (0 NW, 1 N, 2 NE, 3 W, 4 the sample, 5 E, 6 SW, 7 S, 8 SE), which indexes
+// the zero-coding table directly.  A sample found significant in column x
+// updates the scalar copy of its own window (the rows below it in the
+// column) and the window of column x+1 (one lane write); column x-1 and
+// the other stripes rebuild theirs from the flag words when next visited.
+// The sample words of a stripe are updated at its end, with vector
+// instructions, from 64-bit masks of the columns whose row r became
+// significant (or decoded a refinement bit of 0).
+//
+// MQ registers.  A and the Qe values are kept shifted left by 16 (A's bit
+// 31 is T.800's 0x8000), so C is compared and reduced without shifts and
+// RENORMD shifts by the leading zero count at once.  Lane i of `nm` / `nl`
+// holds the context entry that follows index i after an MPS / LPS
+// decision; an entry is Qe << 16 | index | MPS << 6, and `nl` carries the
+// SWITCH flag in bit 6, so the new entry is one select and one XOR.
+
+namespace s0 {
+
+constexpr uint32_t FS = 0x249u;   // significant, rows 0..3
+constexpr uint32_t FV = 0x924u;   // visited, rows 0..3
+constexpr uint32_t FSN = 0x6DBu;  // significant and negative, rows 0..3
+
+struct Lds {
+    uint32_t fl[kMaxStripes][kMaxSide];
+};
+
+__device__ __forceinline__ uint32_t wrlane(uint32_t v, uint32_t lane, uint32_t old) { return wrl(v, lane, old); }
+
+// zero-coding table over the 9-bit window, 4-bit entries, eight per lane
+__device__ __forceinline__ uint32_t build_zc(uint32_t lane, int orient)
+{
+    uint32_t word = 0;
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t i = lane * 8 + j;
+        const uint32_t st = ((i >> 1) & 1 ? S_N : 0) | ((i >> 7) & 1 ? S_S : 0) | ((i >> 3) & 1 ? S_W : 0) |
+                            ((i >> 5) & 1 ? S_E : 0) | (i & 1 ? S_NW : 0) | ((i >> 2) & 1 ? S_NE : 0) |
+                            ((i >> 6) & 1 ? S_SW : 0) | ((i >> 8) & 1 ? S_SE : 0);
+        word |= zc_context(st, orient) << (4 * j);
+    }
+    return word;
+}
+
+// sign-coding table, 8-bit entries (context | xor << 4), four per lane,
+// indexed by the significance of N, W, E, S at bits 1, 3, 5, 7 and their
+// signs at bits 0, 2, 4, 6
+__device__ __forceinline__ uint32_t build_sc(uint32_t lane)
+{
+    uint32_t word = 0;
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t i = lane * 4 + j;
+        const uint32_t st = ((i >> 1) & 1 ? S_N : 0) | ((i >> 3) & 1 ? S_W : 0) | ((i >> 5) & 1 ? S_E : 0) |
+                            ((i >> 7) & 1 ? S_S : 0) | (i & 1 ? S_NNEG : 0) | ((i >> 2) & 1 ? S_WNEG : 0) |
+                            ((i >> 4) & 1 ? S_ENEG : 0) | ((i >> 6) & 1 ? S_SNEG : 0);
+        const uint32_t sc = sc_context(st);
+        word |= ((sc & 0xFF) | ((sc >> 8) << 4)) << (8 * j);
+    }
+    return word;
+}
+
+struct Mq {
+    // MQ registers, A and Qe shifted left by 16
+    uint32_t A, C;
+    int CT;
+    // code stream reader: bytes from the current one on in lo | hi << 32
+    // (nbuf >= 2 of them valid); the next dword is loaded from base + pos
+    uint32_t lo, hi;
+    int nbuf, pos;
+    const uint8_t *base;  // the segment's first byte rounded down to 4
+    int lenx;             // segment length + (segment start - base)
+    uint32_t ctx, nm, nl, zl, sl;
+
+    __device__ __forceinline__ void init(uint32_t lane, int orient)
+    {
+        const uint32_t t = kMqTable[lane < 47 ? lane : 46];
+        const uint32_t ni = (t >> 16) & 63, li = (t >> 22) & 63;
+        nm = ((kMqTable[ni] & 0xFFFFu) << 16) | ni;
+        nl = ((kMqTable[li] & 0xFFFFu) << 16) | li | (((t >> 28) & 1) << 6);
+        const uint32_t k = lane == 0 ? 4u : (lane == CX_RL ? 3u : (lane == CX_UNI ? 46u : 0u));
+        ctx = ((kMqTable[k] & 0xFFFFu) << 16) | k;
+        zl = build_zc(lane, orient);
+        sl = build_sc(lane);
+    }
+    // One DECODE (C.3.2) in context cx with RENORMD and BYTEIN (C.3.3,
+    // C.3.4), in one block of scalar code: the decision chain is wave-
+    // uniform, and as straight-line assembly it keeps every register in
+    // place (compiled as C++, each rare path made the compiler copy the
+    // whole decoder state on the common ones).
+    //   e = the context's entry, qe = Qe << 16; A -= qe;
+    //   C >= qe: C -= qe; A's bit 31 set: MPS, done; else MPS exchange
+    //   (LPS if A < qe);  C < qe: LPS exchange (LPS if A >= qe), A = qe;
+    //   an exchange makes the next entry (nl on LPS, with the MPS switch
+    //   in bit 6, else nm) and RENORMD shifts by A's leading zeros, reading
+    //   a byte whenever CT runs out (two reads cover any shift: after a
+    //   byte that follows 0xFF, which holds 7 bits, comes one of 8).
+    // BYTEIN's temporaries are t, qe and l: m holds the new entry by then.
+    // Lane selects are scalar-ALU results (cx, and e & 63 here), so no
+    // vector-ALU-to-lane-select wait states arise; the one scalar load is
+    // waited for inside the block.
+#define S0_BYTEIN(K)                                              \
+    "s_and_b32 %[t], %[lo], 0xff\n\t"                            \
+    "s_bfe_u32 %[qe], %[lo], 0x80008\n\t"                         \
+    "s_cmp_eq_u32 %[t], 0xff\n\t"                                \
+    "s_cbranch_scc0 .Lb8" K "_%=\n\t"                            \
+    "s_cmp_gt_u32 %[qe], 0x8f\n\t"                                \
+    "s_cbranch_scc0 .Lb9" K "_%=\n\t"                            \
+    "s_add_u32 %[C], %[C], 0xff00\n\t"                           \
+    "s_mov_b32 %[CT], 8\n\t"                                     \
+    "s_branch .Lbe" K "_%=\n"                                    \
+    ".Lb9" K "_%=:\n\t"                                          \
+    "s_lshl_b32 %[qe], %[qe], 9\n\t"                               \
+    "s_add_u32 %[C], %[C], %[qe]\n\t"                             \
+    "s_mov_b32 %[CT], 7\n\t"                                     \
+    "s_branch .Lba" K "_%=\n"                                    \
+    ".Lb8" K "_%=:\n\t"                                          \
+    "s_lshl_b32 %[qe], %[qe], 8\n\t"                               \
+    "s_add_u32 %[C], %[C], %[qe]\n\t"                             \
+    "s_mov_b32 %[CT], 8\n"                                       \
+    ".Lba" K "_%=:\n\t"                                          \
+    "s_lshr_b32 %[lo], %[lo], 8\n\t"                             \
+    "s_lshl_b32 %[t], %[hi], 24\n\t"                             \
+    "s_or_b32 %[lo], %[lo], %[t]\n\t"                            \
+    "s_lshr_b32 %[hi], %[hi], 8\n\t"                             \
+    "s_sub_i32 %[nbuf], %[nbuf], 1\n\t"                          \
+    "s_cmp_gt_i32 %[nbuf], 1\n\t"                                \
+    "s_cbranch_scc1 .Lbe" K "_%=\n\t"                            \
+    "s_load_dword %[t], %[base], %[pos]\n\t"                     \
+    "s_sub_i32 %[qe], %[lenx], %[pos]\n\t"                        \
+    "s_lshl_b32 %[l], %[qe], 3\n\t"                               \
+    "s_lshl_b32 %[l], -1, %[l]\n\t"                              \
+    "s_cmp_lt_i32 %[qe], 4\n\t"                                   \
+    "s_cselect_b32 %[l], %[l], 0\n\t"                            \
+    "s_cmp_le_i32 %[qe], 0\n\t"                                   \
+    "s_cselect_b32 %[l], -1, %[l]\n\t"                           \
+    "s_waitcnt lgkmcnt(0)\n\t"                                   \
+    "s_or_b32 %[t], %[t], %[l]\n\t"                              \
+    "s_lshl_b32 %[l], %[t], 8\n\t"                               \
+    "s_or_b32 %[lo], %[lo], %[l]\n\t"                            \
+    "s_lshr_b32 %[hi], %[t], 24\n\t"                             \
+    "s_add_i32 %[nbuf], %[nbuf], 4\n\t"                          \
+    "s_add_i32 %[pos], %[pos], 4\n"                              \
+    ".Lbe" K "_%=:\n\t"
+
+    // INITDEC (C.3.5) on the block's one codeword segment: the first two
+    // dwords from the segment's start rounded down to 4 (inside the data
+    // buffer, which is 16-byte aligned), bytes past the segment as 0xFF (the
+    // reference appends 0xFF bytes), C = byte 0 << 16, BYTEIN, C <<= 7,
+    // CT -= 7, A = 0x8000.  Later loads reach at most 8 bytes past the
+    // segment's end.
+    __device__ __forceinline__ void start(const uint8_t *sg, int n)
+    {
+        const uintptr_t a = (uintptr_t)sg;
+        const uint32_t sh = (uint32_t)(a & 3);
+        base = (const uint8_t *)(a - sh);
+        lenx = n + (int)sh;
+        uint32_t t, m, l, qe;
+        asm volatile(
+            "s_load_dword %[lo], %[base], 0x0\n\t"
+            "s_load_dword %[hi], %[base], 0x4\n\t"
+            "s_lshl_b32 %[l], %[lenx], 3\n\t"
+            "s_lshl_b32 %[m], -1, %[l]\n\t"
+            "s_cmp_lt_i32 %[lenx], 4\n\t"
+            "s_cselect_b32 %[m], %[m], 0\n\t"
+            "s_cmp_le_i32 %[lenx], 0\n\t"
+            "s_cselect_b32 %[m], -1, %[m]\n\t"
+            "s_sub_i32 %[t], %[lenx], 4\n\t"
+            "s_lshl_b32 %[l], %[t], 3\n\t"
+            "s_lshl_b32 %[l], -1, %[l]\n\t"
+            "s_cmp_lt_i32 %[t], 4\n\t"
+            "s_cselect_b32 %[l], %[l], 0\n\t"
+            "s_cmp_le_i32 %[t], 0\n\t"
+            "s_cselect_b32 %[l], -1, %[l]\n\t"
+            "s_waitcnt lgkmcnt(0)\n\t"
+            "s_or_b32 %[lo], %[lo], %[m]\n\t"
+            "s_or_b32 %[hi], %[hi], %[l]\n\t"
+            "s_lshl_b32 %[l], %[sh], 3\n\t"
+            "s_lshr_b32 %[lo], %[lo], %[l]\n\t"
+            "s_sub_u32 %[m], 32, %[l]\n\t"
+            "s_lshl_b32 %[t], %[hi], %[m]\n\t"
+            "s_cmp_eq_u32 %[sh], 0\n\t"
+            "s_cselect_b32 %[t], 0, %[t]\n\t"
+            "s_or_b32 %[lo], %[lo], %[t]\n\t"
+            "s_lshr_b32 %[hi], %[hi], %[l]\n\t"
+            "s_sub_u32 %[nbuf], 8, %[sh]\n\t"
+            "s_mov_b32 %[pos], 8\n\t"
+            "s_and_b32 %[C], %[lo], 0xff\n\t"
+            "s_lshl_b32 %[C], %[C], 16\n\t"
+            S0_BYTEIN("0")
+            "s_lshl_b32 %[C], %[C], 7\n\t"
+            "s_sub_i32 %[CT], %[CT], 7\n\t"
+            "s_mov_b32 %[A], 0x80000000"
+            : [A] "=&s"(A), [C] "=&s"(C), [CT] "=&s"(CT), [lo] "=&s"(lo), [hi] "=&s"(hi),
+              [nbuf] "=&s"(nbuf), [pos] "=&s"(pos), [t] "=&s"(t), [m] "=&s"(m), [l] "=&s"(l),
+              [qe] "=&s"(qe)
+            : [base] "s"(base), [lenx] "s"(lenx), [sh] "s"(sh)
+            : "scc");
+    }
+
+    __device__ __forceinline__ uint32_t decode(uint32_t cx)
+    {
+        uint32_t d, e, qe, t, m, l;
+        int n;
+        asm volatile(
+            "v_readlane_b32 %[e], %[ctx], %[cx]\n\t"
+            "s_and_b32 %[qe], %[e], 0xffff0000\n\t"
+            "s_sub_u32 %[A], %[A], %[qe]\n\t"
+            "s_bfe_u32 %[d], %[e], 0x10006\n\t"
+            "s_mov_b32 %[m], %[e]\n\t"
+            "s_cmp_lt_u32 %[C], %[qe]\n\t"
+            "s_cbranch_scc1 .Llps_%=\n\t"
+            "s_sub_u32 %[C], %[C], %[qe]\n\t"
+            "s_cmp_lt_i32 %[A], 0\n\t"
+            "s_cbranch_scc1 .Lend_%=\n\t"
+            "s_and_b32 %[t], %[e], 63\n\t"
+            "v_readlane_b32 %[m], %[nm], %[t]\n\t"
+            "v_readlane_b32 %[l], %[nl], %[t]\n\t"
+            "s_and_b32 %[t], %[e], 64\n\t"
+            "s_cmp_lt_u32 %[A], %[qe]\n\t"
+            "s_cselect_b32 %[m], %[l], %[m]\n\t"
+            "s_cselect_b32 %[l], 1, 0\n\t"
+            "s_branch .Lxc_%=\n"
+            ".Llps_%=:\n\t"
+            "s_and_b32 %[t], %[e], 63\n\t"
+            "v_readlane_b32 %[m], %[nm], %[t]\n\t"
+            "v_readlane_b32 %[l], %[nl], %[t]\n\t"
+            "s_and_b32 %[t], %[e], 64\n\t"
+            "s_cmp_ge_u32 %[A], %[qe]\n\t"
+            "s_cselect_b32 %[m], %[l], %[m]\n\t"
+            "s_cselect_b32 %[l], 1, 0\n\t"
+            "s_mov_b32 %[A], %[qe]\n"
+            ".Lxc_%=:\n\t"
+            "s_xor_b32 %[m], %[m], %[t]\n\t"
+            "s_xor_b32 %[d], %[d], %[l]\n\t"
+            "s_flbit_i32_b32 %[n], %[A]\n\t"
+            "s_cmp_gt_i32 %[n], %[CT]\n\t"
+            "s_cbranch_scc1 .Lrd_%=\n"
+            ".Lsh_%=:\n\t"
+            "s_lshl_b32 %[A], %[A], %[n]\n\t"
+            "s_lshl_b32 %[C], %[C], %[n]\n\t"
+            "s_sub_i32 %[CT], %[CT], %[n]\n\t"
+            "s_branch .Lend_%=\n"
+            ".Lrd_%=:\n\t"
+            "s_lshl_b32 %[A], %[A], %[CT]\n\t"
+            "s_lshl_b32 %[C], %[C], %[CT]\n\t"
+            "s_sub_i32 %[n], %[n], %[CT]\n\t"
+            S0_BYTEIN("1")
+            "s_cmp_gt_i32 %[n], %[CT]\n\t"
+            "s_cbranch_scc0 .Lsh_%=\n\t"
+            "s_lshl_b32 %[A], %[A], %[CT]\n\t"
+            "s_lshl_b32 %[C], %[C], %[CT]\n\t"
+            "s_sub_i32 %[n], %[n], %[CT]\n\t"
+            S0_BYTEIN("2")
+            "s_branch .Lsh_%=\n"
+            ".Lend_%=:"
+            : [A] "+s"(A), [C] "+s"(C), [CT] "+s"(CT), [lo] "+s"(lo), [hi] "+s"(hi),
+              [nbuf] "+s"(nbuf), [pos] "+s"(pos), [d] "=&s"(d), [e] "=&s"(e), [qe] "=&s"(qe),
+              [t] "=&s"(t), [m] "=&s"(m), [l] "=&s"(l), [n] "=&s"(n)
+            : [cx] "s"(cx), [ctx] "v"(ctx), [nm] "v"(nm), [nl] "v"(nl), [base] "s"(base),
+              [lenx] "s"(lenx)
+            : "scc");
+        // the context's entry (unchanged unless an exchange happened),
+        // written outside the block: a vector result among the scalar ones
+        // would make the compiler take them all for vector values, and
+        // v_writelane with a scalar value needs its lane in M0 (one more
+        // scalar instruction; the compare and select are vector ones)
+        ctx = wrlane(m, cx, ctx);
+        return d;
+    }
+#undef S0_BYTEIN
+    __device__ __forceinline__ uint32_t zc(uint32_t win)
+    {
+        return (rdl(zl, win >> 3) >> ((win << 2) & 31)) & 0xF;
+    }
+    // sign of a sample whose window of significance is `win` and of signs
+    // (shifted down by one) `nwin`
+    __device__ __forceinline__ uint32_t sign(uint32_t win, uint32_t nwin)
+    {
+        const uint32_t i = (win & 0xAA) | (nwin & 0x55);
+        const uint32_t e = rdl(sl, i >> 2) >> ((i << 3) & 31);
+        return decode(e & 0xF) ^ ((e >> 4) & 1);
+    }
+};
+
+// every column's 3x3 window from `col`, the column's own rows -1..4 at bits
+// 0, 3, ..., 15
+__device__ __forceinline__ uint32_t window(uint32_t col)
+{
+    return (uint32_t)wave_from_left((int)col) | (col << 1) | ((uint32_t)wave_from_right((int)col) << 2);
+}
+__device__ __forceinline__ uint32_t sig_col(uint32_t up, uint32_t cur, uint32_t dn)
+{
+    return ((up >> 9) & 1) | ((cur & FS) << 3) | ((dn & 1) << 15);
+}
+__device__ __forceinline__ uint32_t neg_col(uint32_t up, uint32_t cur, uint32_t dn)
+{
+    return ((up >> 10) & 1) | (((cur >> 1) & FS) << 3) | (((dn >> 1) & 1) << 15);
+}
+// rows of a window's column that are insignificant with a significant
+// neighbour (bit 3r)
+__device__ __forceinline__ uint32_t sig_candidates(uint32_t sw)
+{
+    const uint32_t n3 = sw | (sw >> 1) | (sw >> 2);
+    return (n3 | (n3 >> 3) | (n3 >> 6)) & ~(sw >> 4) & FS;
+}
+
+struct Stripe {
+    uint32_t v0, v1, v2, v3;  // sample words of rows 0..3 (lane = column)
+};
+
+// the sample words of the stripe's new significant samples: sign << 31 |
+// 3 << (plane-1)
+__device__ __forceinline__ void apply_sig(Stripe &v, uint32_t cur, uint32_t lane, uint64_t n0,
+                                          uint64_t n1, uint64_t n2, uint64_t n3, uint32_t one_half)
+{
+    if ((n0 >> lane) & 1) v.v0 = ((cur << 30) & 0x80000000u) + one_half;
+    if ((n1 >> lane) & 1) v.v1 = ((cur << 27) & 0x80000000u) + one_half;
+    if ((n2 >> lane) & 1) v.v2 = ((cur << 24) & 0x80000000u) + one_half;
+    if ((n3 >> lane) & 1) v.v3 = ((cur << 21) & 0x80000000u) + one_half;
+}
+
+// a sample of row R of column x found significant with sign `neg`
+#define S0_SIGNIFICANT(R)                                        \
+    do {                                                         \
+        f |= (1u | (neg << 1)) << (3 * (R));                     \
+        sw |= 1u << (3 * (R) + 4);                               \
+        nw |= neg << (3 * (R) + 4);                              \
+        cs |= 1u << (3 * (R) + 3);                               \
+        cn |= neg << (3 * (R) + 3);                              \
+        ns##R |= 1ull << x;                                      \
+    } while (0)
+
+// significance propagation pass over one stripe
+__device__ __forceinline__ void sigprop(Mq &m, uint32_t &cur, uint32_t up, uint32_t dn, Stripe &v,
+                                        uint32_t rm, int w, uint64_t colmask, uint32_t lane,
+                                        uint32_t one_half)
+{
+    uint32_t SW = window(sig_col(up, cur, dn));
+    uint32_t CD = sig_candidates(SW) & rm;
+    uint64_t todo = __builtin_amdgcn_ballot_w64(CD != 0) & colmask;
+    if (!todo) return;
+    uint32_t NW = window(neg_col(up, cur, dn));
+    uint64_t ns0 = 0, ns1 = 0, ns2 = 0, ns3 = 0;
+    do {
+        const uint32_t x = (uint32_t)__builtin_ctzll(todo);
+        todo &= todo - 1;
+        uint32_t sw = rdl(SW, x), nw = rdl(NW, x), f = rdl(cur, x), c = rdl(CD, x);
+        uint32_t cs = 0, cn = 0;
+#define S0_SP_ROW(R)                                                             \
+        if (c & (1u << (3 * (R)))) {                                             \
+            f |= 4u << (3 * (R));                                                \
+            const uint32_t win = (sw >> (3 * (R))) & 0x1FF;                      \
+            if (m.decode(m.zc(win))) {                                           \
+                const uint32_t neg = m.sign(win, (nw >> (3 * (R) + 1)) & 0xFF);  \
+                S0_SIGNIFICANT(R);                                               \
+                if ((R) < 3) c |= (1u << (3 * (R) + 3)) & ~(sw >> 4) & rm;       \
+            }                                                                    \
+        }
+        S0_SP_ROW(0)
+        S0_SP_ROW(1)
+        S0_SP_ROW(2)
+        S0_SP_ROW(3)
+#undef S0_SP_ROW
+        cur = wrlane(f, x, cur);
+        if (cs && (int)x + 1 < w) {
+            // column x+1 sees the new significance to its west; its rows
+            // next to it may become candidates
+            const uint32_t x1 = x + 1;
+            const uint32_t sw1 = rdl(SW, x1) | cs;
+            SW = wrlane(sw1, x1, SW);
+            NW = wrlane(rdl(NW, x1) | cn, x1, NW);
+            CD = wrlane(sig_candidates(sw1) & rm, x1, CD);
+            todo |= 1ull << x1;
+        }
+    } while (todo);
+    apply_sig(v, cur, lane, ns0, ns1, ns2, ns3, one_half);
+}
+
+// magnitude refinement pass over one stripe
+__device__ __forceinline__ void refine(Mq &m, uint32_t cur, uint32_t up, uint32_t dn, Stripe &v,
+                                       uint64_t colmask, uint32_t lane, int plane, uint32_t half)
+{
+    const uint32_t mu = (cur >> 12) & 0xF;
+    uint64_t todo = __builtin_amdgcn_ballot_w64(mu != 0) & colmask;
+    if (!todo) return;
+    const uint32_t SW = window(sig_col(up, cur, dn));
+    uint64_t z0 = 0, z1 = 0, z2 = 0, z3 = 0;
+    do {
+        const uint32_t x = (uint32_t)__builtin_ctzll(todo);
+        todo &= todo - 1;
+        const uint32_t f = rdl(cur, x), sw = rdl(SW, x);
+// context 16 after the first refinement, else 15 with a significant
+// neighbour, 14 without; written without selects (the compiler may build
+// those with vector instructions, and the decoder needs cx scalar)
+#define S0_MR_ROW(R)                                                                       \
+        if (f & (1u << (12 + (R)))) {                                                      \
+            const uint32_t nb = ((sw & (0x1EFu << (3 * (R)))) + 0x3FFFFu) >> 18;           \
+            const uint32_t cx = CX_MR + max(nb, (f >> (15 + (R))) & 2u);                   \
+            if (!m.decode(cx)) z##R |= 1ull << x;                                          \
+        }
+        S0_MR_ROW(0)
+        S0_MR_ROW(1)
+        S0_MR_ROW(2)
+        S0_MR_ROW(3)
+#undef S0_MR_ROW
+    } while (todo);
+    // bit `plane` cleared where the refinement bit was 0, bit plane-1 set
+    const uint32_t pb = 1u << plane;
+    if (mu & 1) v.v0 = (v.v0 ^ (((z0 >> lane) & 1) ? pb : 0u)) | half;
+    if (mu & 2) v.v1 = (v.v1 ^ (((z1 >> lane) & 1) ? pb : 0u)) | half;
+    if (mu & 4) v.v2 = (v.v2 ^ (((z2 >> lane) & 1) ? pb : 0u)) | half;
+    if (mu & 8) v.v3 = (v.v3 ^ (((z3 >> lane) & 1) ? pb : 0u)) | half;
+}
+
+// cleanup pass over one stripe, then the end-of-plane flag update
+__device__ __forceinline__ void cleanup(Mq &m, uint32_t &cur, uint32_t up, uint32_t dn, Stripe &v,
+                                        uint32_t rm, int w, uint64_t colmask,
+                                        uint32_t lane, uint32_t one_half)
+{
+    // rows not significant and not visited
+    const uint32_t CD = ~(cur | (cur >> 2)) & rm;
+    uint64_t todo = __builtin_amdgcn_ballot_w64(CD != 0) & colmask;
+    if (todo) {
+        uint32_t SW = window(sig_col(up, cur, dn));
+        uint32_t NW = window(neg_col(up, cur, dn));
+        uint64_t ns0 = 0, ns1 = 0, ns2 = 0, ns3 = 0;
+        do {
+            const uint32_t x = (uint32_t)__builtin_ctzll(todo);
+            todo &= todo - 1;
+            uint32_t sw = rdl(SW, x), nw = rdl(NW, x), f = rdl(cur, x), c = rdl(CD, x);
+            uint32_t cs = 0, cn = 0;
+            if ((sw | (c ^ FS)) == 0) {
+                // run mode: four codable samples without significant
+                // neighbours (c == FS needs all four rows, so never in the
+                // short last stripe)
+                if (!m.decode(CX_RL)) continue;
+                uint32_t k = m.decode(CX_UNI) << 1;
+                k |= m.decode(CX_UNI);
+                // its neighbours are all insignificant: context 9, no XOR
+                const uint32_t neg = m.decode(CX_SC);
+                switch (k) {
+                case 0: S0_SIGNIFICANT(0); break;
+                case 1: S0_SIGNIFICANT(1); break;
+                case 2: S0_SIGNIFICANT(2); break;
+                default: S0_SIGNIFICANT(3); break;
+                }
+                c &= ~0u << (3 * k + 3);
+            }
+#define S0_CU_ROW(R)                                                               \
+            if (c & (1u << (3 * (R)))) {                                           \
+                const uint32_t win = (sw >> (3 * (R))) & 0x1FF;                    \
+                if (m.decode(m.zc(win))) {                                         \
+                    const uint32_t neg = m.sign(win, (nw >> (3 * (R) + 1)) & 0xFF); \
+                    S0_SIGNIFICANT(R);                                             \
+                }                                                                  \
+            }
+            S0_CU_ROW(0)
+            S0_CU_ROW(1)
+            S0_CU_ROW(2)
+            S0_CU_ROW(3)
+#undef S0_CU_ROW
+            if (cs) {
+                cur = wrlane(f, x, cur);
+                if ((int)x + 1 < w) {
+                    SW = wrlane(rdl(SW, x + 1) | cs, x + 1, SW);
+                    NW = wrlane(rdl(NW, x + 1) | cn, x + 1, NW);
+                }
+            }
+        } while (todo);
+        apply_sig(v, cur, lane, ns0, ns1, ns2, ns3, one_half);
+    }
+    // end of the bit-plane: "visited" cleared, significant samples become
+    // refinable, refinable ones become "refined before"
+    const uint32_t sig4 = (cur & 1) | ((cur >> 2) & 2) | ((cur >> 4) & 4) | ((cur >> 6) & 8);
+    cur = (cur & FSN) | (sig4 << 12) | (((cur >> 12) & 0xF) << 16);
+}
+#undef S0_SIGNIFICANT
+
+}  // namespace s0
+
+__global__ void __launch_bounds__(64 * kWavesPerGroup)
+    t1_style0_kernel(const BmiT1Block *__restrict__ blocks, int nblocks, const uint8_t *__restrict__ data)
+{
+    __shared__ s0::Lds lds_all[kWavesPerGroup];
+    const uint32_t lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int bi = (int)uni(blockIdx.x * kWavesPerGroup + wave);
+    if (bi >= nblocks) return;
+    s0::Lds &L = lds_all[wave];
+    const BmiT1Block *B = blocks + bi;
+    const int w = (int)uni(B->width), h = (int)uni(B->height);
+    if (w < 1 || h < 1 || w > kMaxSide || h > kMaxSide) return;
+    if (((int)uni(B->style) & BMI_T1_STYLES & ~BMI_T1_PTERM) != 0) return;  // the styled kernel's
+    const int orient = (int)uni(B->orient);
+    const int numbps = (int)uni(B->numbps);
+    const int missing = (int)uni(B->missing);
+    const int num_pass = (int)uni(B->num_pass);
+    const int kind = (int)uni(B->kind);
+    const int len = (int)uni(B->len);
+    const uint64_t off = ((uint64_t)uni((uint32_t)(B->data_offset >> 32)) << 32) |
+                         uni((uint32_t)B->data_offset);
+    const uintptr_t outp = ((uintptr_t)uni((uint32_t)((uintptr_t)B->out >> 32)) << 32) |
+                           uni((uint32_t)(uintptr_t)B->out);
+    const int out_pitch = (int)uni(B->out_pitch);
+    const int nstripes = (h + 3) >> 2;
+    const uint64_t colmask = (w == 64) ? ~0ull : ((1ull << w) - 1);
+
+    auto *o = gptr_w<uint32_t>((void *)outp);
+    const bool col_ok = (int)lane < w;
+    for (int s = 0; s < nstripes; ++s) L.fl[s][lane] = 0;
+    if (col_ok)
+        for (int y = 0; y < h; ++y) o[(size_t)y * out_pitch + lane] = 0;
+    auto vload = [&](int y) -> uint32_t { return (col_ok && y < h) ? o[(size_t)y * out_pitch + lane] : 0u; };
+    auto vstore = [&](int y, uint32_t v) {
+        if (col_ok && y < h) o[(size_t)y * out_pitch + lane] = v;
+    };
+
+    // decode_code_block :610-649 (as t1_decode_kernel)
+    int num_passes = 0;
+    int plane = 0;
+    if ((numbps - missing) * 3 - 2 > 0) {
+        const int p_max = 30 - missing;
+        num_passes = 3 * p_max - 2;
+        if (num_passes > num_pass) num_passes = num_pass;
+        plane = (numbps >> 11) - 1 - missing;
+        if (plane < 0) plane = p_max;
+        if (plane == 0) plane = 1;
+    }
+    // passes below bit-plane 0 are not decoded (a negative shift count,
+    // undefined in the reference): the first cleanup pass is at `plane`,
+    // so at most 3 * plane + 1 passes
+    num_passes = min(num_passes, 3 * plane + 1);
+    if (num_passes > 0) {
+        s0::Mq m;
+        m.init(lane, orient);
+        m.start(data + off, len);
+        // one pass (TYPE 0 significance propagation, 1 refinement, 2
+        // cleanup) over the stripes at bit-plane `plane`
+        auto pass = [&](auto type_tag, int plane) {
+            constexpr int TYPE = decltype(type_tag)::value;
+            const uint32_t one_half = (plane < 31) ? (1u << plane) + ((1u << plane) >> 1) : 0u;
+            const uint32_t half = (plane >= 1 && plane < 32) ? (1u << plane) >> 1 : 0u;
+            uint32_t up = 0, cur = L.fl[0][lane];
+            s0::Stripe v{vload(0), vload(1), vload(2), vload(3)};
+            for (int s = 0; s < nstripes; ++s) {
+                const int rows = min(4, h - 4 * s);
+                const uint32_t rm = rows == 4 ? s0::FS : (s0::FS & ((1u << (3 * rows)) - 1));
+                const uint32_t dn = (s + 1 < nstripes) ? L.fl[s + 1][lane] : 0u;
+                s0::Stripe nx{0, 0, 0, 0};
+                if (s + 1 < nstripes) {
+                    const int y = 4 * (s + 1);
+                    nx = s0::Stripe{vload(y), vload(y + 1), vload(y + 2), vload(y + 3)};
+                }
+                if constexpr (TYPE == 0) s0::sigprop(m, cur, up, dn, v, rm, w, colmask, lane, one_half);
+                if constexpr (TYPE == 1) s0::refine(m, cur, up, dn, v, colmask, lane, plane, half);
+                if constexpr (TYPE == 2) s0::cleanup(m, cur, up, dn, v, rm, w, colmask, lane, one_half);
+                if constexpr (TYPE != 1) L.fl[s][lane] = cur;
+                vstore(4 * s, v.v0);
+                vstore(4 * s + 1, v.v1);
+                vstore(4 * s + 2, v.v2);
+                vstore(4 * s + 3, v.v3);
+                up = cur;
+                cur = dn;
+                v = nx;
+            }
+        };
+        // a cleanup pass at the start plane, then significance propagation,
+        // refinement and cleanup a plane lower each time (written without
+        // a three-way branch per pass, which the compiler handles badly
+        // around the scalar decoder state)
+        int rest = num_passes;
+        for (int it = 0; rest > 0; ++it) {
+            if (it > 0) {
+                --plane;
+                pass(std::integral_constant<int, 0>{}, plane);
+                if (--rest > 0) {
+                    pass(std::integral_constant<int, 1>{}, plane);
+                    --rest;
+                }
+            }
+            if (rest > 0) {
+                pass(std::integral_constant<int, 2>{}, plane);
+                --rest;
+            }
+        }
+    }
+    // 5/3: convert the words in place (dec_cb_stripe, dec_t1_comp.cpp:343-367)
+    if (kind == BMI_T1_INT && col_ok) {
+        const int sh = 31 - numbps;
+        for (int y = 0; y < h; ++y) {
+            const uint32_t v = o[(size_t)y * out_pitch + lane];
+            const int32_t r = (v & 0x80000000u) ? -(int32_t)((v & 0x7FFFFFFFu) >> sh) : (int32_t)(v >> sh);
+            o[(size_t)y * out_pitch + lane] = (uint32_t)r;
+        }
+    }
+}
+
 }  // namespace
 }  // namespace bmi
 
@@ -766,7 +1382,7 @@ extern "C" int BMI_t1_decode_blocks(const BmiT1Block *blocks, int nblocks, const
     // the check below must see this launch, not an error left by an
     // earlier, unrelated runtime call of the caller
     (void)hipGetLastError();
-    hipLaunchKernelGGL(t1_decode_kernel<false>, grid, block, 0, (hipStream_t)stream, blocks, nblocks,
+    hipLaunchKernelGGL(t1_style0_kernel, grid, block, 0, (hipStream_t)stream, blocks, nblocks,
                        (const uint8_t *)data);
     hipLaunchKernelGGL(t1_decode_kernel<true>, grid, block, 0, (hipStream_t)stream, blocks, nblocks,
                        (const uint8_t *)data);
