@@ -2,32 +2,39 @@
 // per wavefront.
 //
 // The MQ decoder is one serial chain of decisions per code-block, so a
-// wave runs it as wave-uniform code: every value of the chain (MQ
-// registers, context states, the current sample's state word) is uniform
-// and lives in scalar registers, and the branches are scalar.  The 64
-// lanes do the parallel parts: clearing the block state, and writing the
-// decoded block (one lane per column, coalesced rows).
+// wave runs it as wave-uniform code: the MQ registers, context states and
+// the current column's state are uniform and live in scalar registers, and
+// the branches are scalar.  The 64 lanes do the parallel parts, one lane
+// per column: clearing the block state, building per-stripe state, and
+// reading and writing the decoded block (coalesced rows).
 //
-// The stripe being decoded lives in registers, one column per lane: four
-// sample words and four 16-bit state words.  Across stripes, the state
-// words are kept in LDS (8 KB per wave) and the sample words in the block's
-// own output rows in global memory, loaded one stripe ahead of use and
-// stored back after it.  The state word carries the significance of the
-// eight neighbours, so a context is formed from one lookup:
-//   bits 0..7   significance of N, S, W, E, NW, NE, SW, SE
-//   bit 8       significant        bit 9   negative
-//   bit 10      coded in this plane's significance pass
-//   bit 11      refinable (significant before the current plane)
-//   bits 12..15 sign of the N, S, W, E neighbours
+// Two kernels run over the same block table, each skipping the other's
+// blocks:
+//  - t1_style0_kernel, for code-block style 0 (the style the reference
+//    decodes): per stripe and column a flag word in LDS, 3x3 significance
+//    windows built by the lanes at each stripe, and the MQ decoder as one
+//    block of scalar assembly (its comments below);
+//  - t1_styled_kernel, for every other switch of T.800 Table A.19
+//    (bypass with raw passes, context reset, termination on every pass,
+//    vertically causal contexts, segmentation symbols; D.5-D.7), which the
+//    reference asserts on.  There the stripe being decoded lives in
+//    registers, one column per lane: four sample words and four 16-bit
+//    state words; across stripes the state words are kept in LDS (8 KB per
+//    wave) and the sample words in the block's own output rows in global
+//    memory, loaded one stripe ahead of use and stored back after it.  The
+//    state word carries the significance of the eight neighbours, so a
+//    context is formed from one lookup:
+//      bits 0..7   significance of N, S, W, E, NW, NE, SW, SE
+//      bit 8       significant        bit 9   negative
+//      bit 10      coded in this plane's significance pass
+//      bit 11      refinable (significant before the current plane)
+//      bits 12..15 sign of the N, S, W, E neighbours
 //
 // Semantics follow decode_code_block (decoder/codec/src/bmi_block_decoder.cpp:
 // 589-853) and its passes (:10-586), the MQ decoder
 // (decoder/codec/src/bmi_mq_decoder.cpp:186-355) and dec_cb_stripe's copy
 // (decoder/codec/src/dec_t1_comp.cpp:283-426); the procedures are written
-// from ITU-T T.800 C.3 and D.  Code-block styles other than 0 (bypass with raw
-// passes, context reset, termination on every pass, vertically causal
-// contexts, segmentation symbols: T.800 Table A.19, D.5-D.7), which the
-// reference asserts on, run in a second instantiation of the kernel.
+// from ITU-T T.800 C.3 and D.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -490,16 +497,15 @@ __device__ __forceinline__ void rows_mask(int rows, uint32_t bits, uint32_t &mlo
     mhi = (rows > 2 ? bits : 0) | (rows > 3 ? bits << 16 : 0);
 }
 
-// Two instantiations, launched one after the other over the same table:
-// STYLED false decodes the blocks of the code-block style the reference
-// decodes (0, or predictable termination), STYLED true those with any
-// other switch of T.800 Table A.19; each skips the other's blocks.  Kept
-// apart so that the style machinery costs the common kernel no registers.
-template <bool STYLED>
+// The blocks with any switch of T.800 Table A.19 other than predictable
+// termination; t1_style0_kernel, launched over the same table, decodes the
+// others, and each skips the other's blocks.  Kept apart so that the style
+// machinery costs the common kernel no registers.
 __global__ void __launch_bounds__(64 * kWavesPerGroup)
-    t1_decode_kernel(const BmiT1Block *__restrict__ blocks, int nblocks,
+    t1_styled_kernel(const BmiT1Block *__restrict__ blocks, int nblocks,
                      const uint8_t *__restrict__ data)
 {
+    constexpr bool STYLED = true;
     __shared__ WaveLds lds_all[kWavesPerGroup];
     const uint32_t lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -819,14 +825,16 @@ struct Mq {
     // uniform, and as straight-line assembly it keeps every register in
     // place (compiled as C++, each rare path made the compiler copy the
     // whole decoder state on the common ones).
-    //   e = the context's entry, qe = Qe << 16; A -= qe;
-    //   C >= qe: C -= qe; A's bit 31 set: MPS, done; else MPS exchange
-    //   (LPS if A < qe);  C < qe: LPS exchange (LPS if A >= qe), A = qe;
-    //   an exchange makes the next entry (nl on LPS, with the MPS switch
+    //   e = the context's entry, qe = Qe << 16;
+    //   C >= qe (no borrow from C -= qe): A -= qe; A's bit 31 still set
+    //   (no signed overflow): MPS, done; else MPS exchange (LPS if A < qe);
+    //   C < qe: C restored, A -= qe, LPS exchange (LPS if A >= qe), A = qe;
+    //   an exchange makes e the next entry (nl on LPS, with the MPS switch
     //   in bit 6, else nm) and RENORMD shifts by A's leading zeros, reading
     //   a byte whenever CT runs out (two reads cover any shift: after a
     //   byte that follows 0xFF, which holds 7 bits, comes one of 8).
-    // BYTEIN's temporaries are t, qe and l: m holds the new entry by then.
+    // The entry is written back after the block (unchanged on the MPS
+    // path).  BYTEIN's temporaries are t, qe and l.
     // Lane selects are scalar-ALU results (cx, and e & 63 here), so no
     // vector-ALU-to-lane-select wait states arise; the one scalar load is
     // waited for inside the block.
@@ -931,39 +939,41 @@ struct Mq {
 
     __device__ __forceinline__ uint32_t decode(uint32_t cx)
     {
-        uint32_t d, e, qe, t, m, l;
+        uint32_t d, e, qe, t, l;
         int n;
         asm volatile(
             "v_readlane_b32 %[e], %[ctx], %[cx]\n\t"
             "s_and_b32 %[qe], %[e], 0xffff0000\n\t"
-            "s_sub_u32 %[A], %[A], %[qe]\n\t"
             "s_bfe_u32 %[d], %[e], 0x10006\n\t"
-            "s_mov_b32 %[m], %[e]\n\t"
-            "s_cmp_lt_u32 %[C], %[qe]\n\t"
-            "s_cbranch_scc1 .Llps_%=\n\t"
+            // SCC: borrow, C < qe
             "s_sub_u32 %[C], %[C], %[qe]\n\t"
-            "s_cmp_lt_i32 %[A], 0\n\t"
-            "s_cbranch_scc1 .Lend_%=\n\t"
+            "s_cbranch_scc1 .Llps_%=\n\t"
+            // SCC: signed overflow, A's bit 31 now clear
+            "s_sub_i32 %[A], %[A], %[qe]\n\t"
+            "s_cbranch_scc0 .Lend_%=\n\t"
+            // MPS exchange: the LPS if A < qe
             "s_and_b32 %[t], %[e], 63\n\t"
-            "v_readlane_b32 %[m], %[nm], %[t]\n\t"
-            "v_readlane_b32 %[l], %[nl], %[t]\n\t"
-            "s_and_b32 %[t], %[e], 64\n\t"
             "s_cmp_lt_u32 %[A], %[qe]\n\t"
-            "s_cselect_b32 %[m], %[l], %[m]\n\t"
-            "s_cselect_b32 %[l], 1, 0\n\t"
-            "s_branch .Lxc_%=\n"
+            "s_cbranch_scc1 .Lxl_%=\n"
+            ".Lxm_%=:\n\t"
+            "v_readlane_b32 %[l], %[nm], %[t]\n\t"
+            "s_and_b32 %[t], %[e], 64\n\t"
+            "s_or_b32 %[e], %[l], %[t]\n\t"
+            "s_branch .Lrn_%=\n"
             ".Llps_%=:\n\t"
+            "s_add_u32 %[C], %[C], %[qe]\n\t"
+            "s_sub_u32 %[A], %[A], %[qe]\n\t"
             "s_and_b32 %[t], %[e], 63\n\t"
-            "v_readlane_b32 %[m], %[nm], %[t]\n\t"
+            // LPS exchange: the LPS if A >= qe
+            "s_cmp_ge_u32 %[A], %[qe]\n\t"
+            "s_mov_b32 %[A], %[qe]\n\t"
+            "s_cbranch_scc0 .Lxm_%=\n"
+            ".Lxl_%=:\n\t"
             "v_readlane_b32 %[l], %[nl], %[t]\n\t"
             "s_and_b32 %[t], %[e], 64\n\t"
-            "s_cmp_ge_u32 %[A], %[qe]\n\t"
-            "s_cselect_b32 %[m], %[l], %[m]\n\t"
-            "s_cselect_b32 %[l], 1, 0\n\t"
-            "s_mov_b32 %[A], %[qe]\n"
-            ".Lxc_%=:\n\t"
-            "s_xor_b32 %[m], %[m], %[t]\n\t"
-            "s_xor_b32 %[d], %[d], %[l]\n\t"
+            "s_xor_b32 %[e], %[l], %[t]\n\t"
+            "s_xor_b32 %[d], %[d], 1\n"
+            ".Lrn_%=:\n\t"
             "s_flbit_i32_b32 %[n], %[A]\n\t"
             "s_cmp_gt_i32 %[n], %[CT]\n\t"
             "s_cbranch_scc1 .Lrd_%=\n"
@@ -987,7 +997,7 @@ struct Mq {
             ".Lend_%=:"
             : [A] "+s"(A), [C] "+s"(C), [CT] "+s"(CT), [lo] "+s"(lo), [hi] "+s"(hi),
               [nbuf] "+s"(nbuf), [pos] "+s"(pos), [d] "=&s"(d), [e] "=&s"(e), [qe] "=&s"(qe),
-              [t] "=&s"(t), [m] "=&s"(m), [l] "=&s"(l), [n] "=&s"(n)
+              [t] "=&s"(t), [l] "=&s"(l), [n] "=&s"(n)
             : [cx] "s"(cx), [ctx] "v"(ctx), [nm] "v"(nm), [nl] "v"(nl), [base] "s"(base),
               [lenx] "s"(lenx)
             : "scc");
@@ -996,7 +1006,7 @@ struct Mq {
         // would make the compiler take them all for vector values, and
         // v_writelane with a scalar value needs its lane in M0 (one more
         // scalar instruction; the compare and select are vector ones)
-        ctx = wrlane(m, cx, ctx);
+        ctx = wrlane(e, cx, ctx);
         return d;
     }
 #undef S0_BYTEIN
@@ -1247,7 +1257,7 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup)
         if (col_ok && y < h) o[(size_t)y * out_pitch + lane] = v;
     };
 
-    // decode_code_block :610-649 (as t1_decode_kernel)
+    // decode_code_block :610-649 (as t1_styled_kernel)
     int num_passes = 0;
     int plane = 0;
     if ((numbps - missing) * 3 - 2 > 0) {
@@ -1384,7 +1394,7 @@ extern "C" int BMI_t1_decode_blocks(const BmiT1Block *blocks, int nblocks, const
     (void)hipGetLastError();
     hipLaunchKernelGGL(t1_style0_kernel, grid, block, 0, (hipStream_t)stream, blocks, nblocks,
                        (const uint8_t *)data);
-    hipLaunchKernelGGL(t1_decode_kernel<true>, grid, block, 0, (hipStream_t)stream, blocks, nblocks,
+    hipLaunchKernelGGL(t1_styled_kernel, grid, block, 0, (hipStream_t)stream, blocks, nblocks,
                        (const uint8_t *)data);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return bmi_fail(BMI_E_HIP, "t1 launch: %s", hipGetErrorString(e));
