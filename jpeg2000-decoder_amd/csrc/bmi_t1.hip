@@ -1085,7 +1085,7 @@ __device__ __forceinline__ void sigprop(Mq &m, uint32_t &cur, uint32_t up, uint3
     uint64_t ns0 = 0, ns1 = 0, ns2 = 0, ns3 = 0;
     do {
         const uint32_t x = (uint32_t)__builtin_ctzll(todo);
-        todo &= todo - 1;
+        todo &= ~(1ull << x);
         uint32_t sw = rdl(SW, x), nw = rdl(NW, x), f = rdl(cur, x), c = rdl(CD, x);
         uint32_t cs = 0, cn = 0;
 #define S0_SP_ROW(R)                                                             \
@@ -1118,7 +1118,11 @@ __device__ __forceinline__ void sigprop(Mq &m, uint32_t &cur, uint32_t up, uint3
     apply_sig(v, cur, lane, ns0, ns1, ns2, ns3, one_half);
 }
 
-// magnitude refinement pass over one stripe
+// magnitude refinement pass over one stripe.  The stripe's significance
+// does not change in this pass, so each refinable sample's context is
+// known at the start: the lanes compute it (16 after the first
+// refinement, else 15 with a significant neighbour, 14 without; 0 where
+// the sample is not refinable) for the four rows, five bits each.
 __device__ __forceinline__ void refine(Mq &m, uint32_t cur, uint32_t up, uint32_t dn, Stripe &v,
                                        uint64_t colmask, uint32_t lane, int plane, uint32_t half)
 {
@@ -1126,19 +1130,21 @@ __device__ __forceinline__ void refine(Mq &m, uint32_t cur, uint32_t up, uint32_
     uint64_t todo = __builtin_amdgcn_ballot_w64(mu != 0) & colmask;
     if (!todo) return;
     const uint32_t SW = window(sig_col(up, cur, dn));
+    uint32_t MC = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t c = ((cur >> (16 + r)) & 1) ? CX_MR + 2u
+                                                   : (((SW >> (3 * r)) & 0x1EF) ? CX_MR + 1u : CX_MR + 0u);
+        MC |= (((mu >> r) & 1) ? c : 0u) << (5 * r);
+    }
     uint64_t z0 = 0, z1 = 0, z2 = 0, z3 = 0;
     do {
         const uint32_t x = (uint32_t)__builtin_ctzll(todo);
-        todo &= todo - 1;
-        const uint32_t f = rdl(cur, x), sw = rdl(SW, x);
-// context 16 after the first refinement, else 15 with a significant
-// neighbour, 14 without; written without selects (the compiler may build
-// those with vector instructions, and the decoder needs cx scalar)
-#define S0_MR_ROW(R)                                                                       \
-        if (f & (1u << (12 + (R)))) {                                                      \
-            const uint32_t nb = ((sw & (0x1EFu << (3 * (R)))) + 0x3FFFFu) >> 18;           \
-            const uint32_t cx = CX_MR + max(nb, (f >> (15 + (R))) & 2u);                   \
-            if (!m.decode(cx)) z##R |= 1ull << x;                                          \
+        todo &= ~(1ull << x);
+        const uint32_t mc = rdl(MC, x);
+#define S0_MR_ROW(R)                                                  \
+        if (const uint32_t cx = (mc >> (5 * (R))) & 31) {             \
+            if (!m.decode(cx)) z##R |= 1ull << x;                     \
         }
         S0_MR_ROW(0)
         S0_MR_ROW(1)
@@ -1168,7 +1174,7 @@ __device__ __forceinline__ void cleanup(Mq &m, uint32_t &cur, uint32_t up, uint3
         uint64_t ns0 = 0, ns1 = 0, ns2 = 0, ns3 = 0;
         do {
             const uint32_t x = (uint32_t)__builtin_ctzll(todo);
-            todo &= todo - 1;
+            todo &= ~(1ull << x);
             uint32_t sw = rdl(SW, x), nw = rdl(NW, x), f = rdl(cur, x), c = rdl(CD, x);
             uint32_t cs = 0, cn = 0;
             if ((sw | (c ^ FS)) == 0) {
