@@ -1090,7 +1090,6 @@ __device__ __forceinline__ void sigprop(Mq &m, uint32_t &cur, uint32_t up, uint3
         uint32_t cs = 0, cn = 0;
 #define S0_SP_ROW(R)                                                             \
         if (c & (1u << (3 * (R)))) {                                             \
-            f |= 4u << (3 * (R));                                                \
             const uint32_t win = (sw >> (3 * (R))) & 0x1FF;                      \
             if (m.decode(m.zc(win))) {                                           \
                 const uint32_t neg = m.sign(win, (nw >> (3 * (R) + 1)) & 0xFF);  \
@@ -1103,15 +1102,16 @@ __device__ __forceinline__ void sigprop(Mq &m, uint32_t &cur, uint32_t up, uint3
         S0_SP_ROW(2)
         S0_SP_ROW(3)
 #undef S0_SP_ROW
-        cur = wrlane(f, x, cur);
+        // every candidate row was coded: "visited"
+        cur = wrlane(f | (c << 2), x, cur);
         if (cs && (int)x + 1 < w) {
             // column x+1 sees the new significance to its west; its rows
-            // next to it may become candidates
+            // next to it may become candidates (recomputed by the lanes:
+            // the columns up to x are done, the others unchanged)
             const uint32_t x1 = x + 1;
-            const uint32_t sw1 = rdl(SW, x1) | cs;
-            SW = wrlane(sw1, x1, SW);
+            SW = wrlane(rdl(SW, x1) | cs, x1, SW);
             NW = wrlane(rdl(NW, x1) | cn, x1, NW);
-            CD = wrlane(sig_candidates(sw1) & rm, x1, CD);
+            CD = sig_candidates(SW) & rm;
             todo |= 1ull << x1;
         }
     } while (todo);
