@@ -197,7 +197,7 @@ def batched_leg(pkg, wl, torch, dev, stream, steps, warmup, inflight=2):
            "us_one_at_a_time_by_launch_mode": {k: round(v * 1e3, 1) for k, v in one_ms.items()},
            "tuning": {"seg_rows": best.seg_rows, "waves_per_group": best.waves_per_group,
                       "min_waves_per_cu": best.min_waves_per_cu, "persistent": best.persistent,
-                      "fuse2": best.fuse2},
+                      "fuse2": best.fuse2, "from_coarse": best.from_coarse},
            "autotune_s": round(tune_s, 2)}
     for p_ in plans:
         p_.close()
@@ -468,8 +468,10 @@ def main() -> int:
     ap.add_argument("--sets", type=int, default=8,
                     help="coefficient / output buffer sets the timed images rotate over "
                          "(8 x 133 MB: more than the 256 MB last-level cache)")
-    ap.add_argument("--images-per-plan", type=int, default=4,
+    ap.add_argument("--images-per-plan", type=int, default=8,
                     help="images per plan in the throughput (reported) form")
+    ap.add_argument("--batch-inflight", type=int, default=3,
+                    help="batched leg: plans in flight on their own streams")
     ap.add_argument("--inflight", type=int, default=2,
                     help="plans in flight at once, each on its own stream")
     ap.add_argument("--legs", default="single,plain,batched,serving,t1,cpu",
@@ -735,7 +737,8 @@ def main() -> int:
     batched = None
     if "batched" in legs:
         log("batched: 256 mixed tiles, autotuned, hipGraph")
-        batched = batched_leg(pkg, wl, torch, dev, stream, max(5, args.steps // 5), 2)
+        batched = batched_leg(pkg, wl, torch, dev, stream, max(5, args.steps // 5), 2,
+                              inflight=args.batch_inflight)
         log(f"batched: {batched['ms_per_batch']} ms, {batched['Mpixel_s']} Mpixel/s, "
             f"{batched['frac_of_peak'] * 100:.1f}% of peak; plain {batched['plain_Mpixel_s']} "
             f"Mpixel/s ({batched['speedup_vs_plain']}x)")

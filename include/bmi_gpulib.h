@@ -186,9 +186,15 @@ typedef struct BmiTuning {
                             seg_rows output rows (the cost model is off) */
     int fuse2;           /* the last two 9/7 synthesis levels of a tile in
                             one pass (the lower one kept in LDS, never in
-                            HBM): 0 default (on), -1 off.  Step modes 0 and
-                            3 only; persistent and all-merged modes do not
-                            fuse */
+                            HBM): 0 default (on), -1 off, N > 0 on for tiles
+                            whose final level spans at least N samples each
+                            way.  Step modes 0 and 3 only; persistent and
+                            all-merged modes do not fuse */
+    int from_coarse;     /* 0 (default): level steps count from each
+                            tile's finest level (the last step holds every
+                            tile's final level, so all two-level launches
+                            fall in one step); 1: from the coarsest (step k
+                            holds every tile's level k) */
 } BmiTuning;
 
 typedef struct BmiBatchPlan BmiBatchPlan;

@@ -96,6 +96,7 @@ class BmiTuning(ctypes.Structure):
         ("min_waves_per_cu", ctypes.c_int),
         ("fixed_seg", ctypes.c_int),
         ("fuse2", ctypes.c_int),
+        ("from_coarse", ctypes.c_int),
     ]
 
 

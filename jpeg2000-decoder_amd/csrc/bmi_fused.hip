@@ -1456,7 +1456,8 @@ struct BmiBatchPlan {
     std::vector<Launch> launches;  // level-step mode: per step and variant
     std::vector<int> job_tile;                // tile of each job
     struct {
-        int seg_rows, waves_per_group, groups_per_cu, persistent, min_waves_per_cu, fixed_seg, fuse2;
+        int seg_rows, waves_per_group, groups_per_cu, persistent, min_waves_per_cu, fixed_seg, fuse2,
+            from_coarse;
     } tuning;
     std::vector<char> tile_fused;  // per tile: its last two levels run as one two-level step
     LevelJob *d_jobs = nullptr;
@@ -1589,9 +1590,10 @@ int bmi_apply_ignore(const BmiTileDesc &in, BmiTileDesc *out)
 }
 
 // A tile runs its last two synthesis levels as one two-level step when
-// both levels are 9/7, at least two samples in each direction, and a
-// two-level kernel exists for its final level's variant.
-static bool tile_fuses(const BmiTileDesc &t)
+// both levels are 9/7, at least two samples in each direction, its final
+// level spans at least `min_side` samples each way, and a two-level kernel
+// exists for its final level's variant.
+static bool tile_fuses(const BmiTileDesc &t, int min_side)
 {
     if (t.method != Ckernels_W9X7) return false;
     const int L = t.levels;
@@ -1602,6 +1604,7 @@ static bool tile_fuses(const BmiTileDesc &t)
         level_span(t.x0, t.width, L, lev, &a0, &a1);
         level_span(t.y0, t.height, L, lev, &b0, &b1);
         if (a1 - a0 < 2 || b1 - b0 < 2) return false;
+        if (lev == Ls - 1 && (a1 - a0 < min_side || b1 - b0 < min_side)) return false;
     }
     const int ok = out_kind_for(t);
     int comps[4], n = 0;
@@ -1651,7 +1654,8 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
     P->tuning.persistent = tuning ? tuning->persistent : 0;
     P->tuning.min_waves_per_cu = (tuning && tuning->min_waves_per_cu > 0) ? tuning->min_waves_per_cu : 4;
     P->tuning.fixed_seg = (tuning && tuning->fixed_seg) ? 1 : 0;
-    P->tuning.fuse2 = (tuning && tuning->fuse2 < 0) ? -1 : 0;
+    P->tuning.fuse2 = tuning ? (tuning->fuse2 < 0 ? -1 : tuning->fuse2) : 0;
+    P->tuning.from_coarse = (tuning && tuning->from_coarse) ? 1 : 0;
     if (P->tuning.waves_per_group > 4) P->tuning.waves_per_group = 4;
     if (hipGetDevice(&P->device) != hipSuccess) P->device = 0;
     {
@@ -1665,7 +1669,8 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
     // which tiles run their last two levels as one two-level step
     P->tile_fused.assign(ntiles, 0);
     const bool fuse_mode = P->tuning.fuse2 >= 0 && P->tuning.persistent != 1 && P->tuning.persistent != 2;
-    for (int ti = 0; ti < ntiles && fuse_mode; ++ti) P->tile_fused[ti] = tile_fuses(tiles[ti]) ? 1 : 0;
+    for (int ti = 0; ti < ntiles && fuse_mode; ++ti)
+        P->tile_fused[ti] = tile_fuses(tiles[ti], P->tuning.fuse2) ? 1 : 0;
 
     // scratch layout: per tile, per needed component, two planes sized for
     // the largest intermediate level written to HBM (a two-level step keeps
@@ -1716,7 +1721,11 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
             const BmiTileDesc &t = tiles[ti];
             const int L = t.levels;
             const int Ls = (t.top_level < 0) ? L : t.top_level;
-            const int lev = step;
+            // the tile's level in this step: steps aligned at every tile's
+            // finest level (by default: the final levels, most of the work,
+            // share the last step; 256 mixed tiles 0.49 -> 0.43 ms) or at
+            // the coarsest
+            const int lev = P->tuning.from_coarse ? step : step - (max_steps - Ls);
             if (Ls == 0 && step == 0) {
                 // thumbnail: the LL band of the coarsest level, no synthesis
                 const int filt = (t.method == Ckernels_W9X7) ? 97 : 53;
@@ -1774,6 +1783,7 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                 continue;
             }
             if (lev >= Ls) continue;
+            if (lev < 0) continue;
             const bool fused = P->tile_fused[ti] != 0;
             if (fused && lev == Ls - 2) continue;  // synthesised inside the final level's items
             const bool final_level = (lev == Ls - 1);
@@ -1938,8 +1948,8 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                 push_job(J, -1);
             }
         }
-        // Segment height of the step: the same number of row pairs for every
-        // job, chosen by a cost model of the launch.  When the step's items
+        // Segment height of a launch: the same number of row pairs for every
+        // job in it, chosen by a cost model of the launch.  When the items
         // at the tuned height fit on the GPU at once (at most four waves per
         // SIMD), they all run side by side and the step lasts about as long
         // as its busiest SIMD: rounds = ceil(items / SIMDs) waves (at least
@@ -1953,25 +1963,38 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
         // some SIMDs three) in 34.4 us.  Larger steps, which the dispatcher
         // refills as waves finish, search only up to the tuned height (256
         // mixed tiles: 0.49 ms at 32 rows, 0.50 ms with 64 allowed).
-        // Thumbnail jobs keep their one-pair items.
+        // Thumbnail jobs keep their one-pair items.  The launches of a step
+        // are the jobs the dispatching kernel (or one specialised kernel)
+        // takes, and each two-level variant, which launches on its own.
         {
             const long target = (long)P->num_cus * P->tuning.min_waves_per_cu;
             const long simds = 4L * P->num_cus;
-            long fixed = 0;
-            bool any97 = false, any = false, any2 = false;
-            int max_pairs = 1;  // row pairs of the tallest job: no taller segment helps
+            const int mode = P->tuning.persistent;
+            auto group_of = [&](const LevelJob &J) {
+                return (J.fuse2 || mode == 3) ? variant_of(J) : -1;
+            };
+            std::vector<int> groups;
             for (int j = step_job0; j < (int)P->jobs.size(); ++j) {
-                const LevelJob &J = P->jobs[j];
-                if (J.seg_pairs <= 1) {
-                    fixed += J.n_items;
-                } else {
-                    any = true;
-                    any97 = any97 || J.filt == 97;
-                    any2 = any2 || J.fuse2;
-                    max_pairs = std::max(max_pairs, (J.b1 - (J.b0 & ~1) + 1) / 2);
-                }
+                const int g = group_of(P->jobs[j]);
+                if (std::find(groups.begin(), groups.end(), g) == groups.end()) groups.push_back(g);
             }
-            if (any && !P->tuning.fixed_seg) {
+            for (const int g : groups) {
+                long fixed = 0;
+                bool any97 = false, any = false, any2 = false;
+                int max_pairs = 1;  // row pairs of the tallest job: no taller segment helps
+                for (int j = step_job0; j < (int)P->jobs.size(); ++j) {
+                    const LevelJob &J = P->jobs[j];
+                    if (group_of(J) != g) continue;
+                    if (J.seg_pairs <= 1) {
+                        fixed += J.n_items;
+                    } else {
+                        any = true;
+                        any97 = any97 || J.filt == 97;
+                        any2 = any2 || J.fuse2;
+                        max_pairs = std::max(max_pairs, (J.b1 - (J.b0 & ~1) + 1) / 2);
+                    }
+                }
+                if (!any || P->tuning.fixed_seg) continue;
                 // a two-level item also re-synthesises about 8 rows' worth
                 // of the level below at each end of its segment
                 const int halo = any2 ? 16 : (any97 ? 8 : 4);
@@ -1979,7 +2002,7 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                     long total = fixed;
                     for (int j = step_job0; j < (int)P->jobs.size(); ++j) {
                         const LevelJob &J = P->jobs[j];
-                        if (J.seg_pairs <= 1) continue;
+                        if (J.seg_pairs <= 1 || group_of(J) != g) continue;
                         const int nrows = J.b1 - (J.b0 & ~1);
                         total += (long)J.nstrips * ((nrows + 2 * sp - 1) / (2 * sp));
                     }
@@ -2007,7 +2030,7 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                 }
                 for (int j = step_job0; j < (int)P->jobs.size(); ++j) {
                     LevelJob &J = P->jobs[j];
-                    if (J.seg_pairs <= 1) continue;
+                    if (J.seg_pairs <= 1 || group_of(J) != g) continue;
                     J.seg_pairs = best;
                     const int nrows = J.b1 - (J.b0 & ~1);
                     J.nsegs = (nrows + 2 * J.seg_pairs - 1) / (2 * J.seg_pairs);
@@ -2454,12 +2477,16 @@ extern "C" int BMI_autotune(const BmiTileDesc *tiles, int ntiles, const BmiTunin
     } else {
         // default grid: segment height x items per CU x waves per group x
         // (merged or per-variant launches of steps that mix variants) x
-        // (the last two 9/7 levels of a tile in one pass or two)
+        // (the last two 9/7 levels of a tile in one pass, for tiles of any
+        // size or of final levels of 256 samples and more, or in two) x
+        // (level steps aligned at the finest or the coarsest level)
         for (int seg : {16, 32, 64, 128})
             for (int mw : {4, 8, 16})
-                for (int wpg : {0, 2, 4})   // 0: chosen per launch
+                for (int wpg : {0, 4})   // 0: chosen per launch
                     for (int mode : {0, 3})
-                        for (int fuse : {0, -1}) cand.push_back(BmiTuning{seg, wpg, 0, mode, mw, 0, fuse});
+                        for (int fuse : {0, 256, -1})
+                            for (int coarse : {0, 1})
+                                cand.push_back(BmiTuning{seg, wpg, 0, mode, mw, 0, fuse, coarse});
     }
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t e0, e1;

@@ -757,7 +757,6 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup)
 namespace s0 {
 
 constexpr uint32_t FS = 0x249u;   // significant, rows 0..3
-constexpr uint32_t FV = 0x924u;   // visited, rows 0..3
 constexpr uint32_t FSN = 0x6DBu;  // significant and negative, rows 0..3
 
 struct Lds {

@@ -48,7 +48,7 @@ def test_struct_layouts(pkg):
     assert d.coef_pitch.offset == 32
     assert d.steps.offset == 92
     assert d.out.offset == 400
-    assert ctypes.sizeof(pkg.BmiTuning) == 28
+    assert ctypes.sizeof(pkg.BmiTuning) == 32
 
 
 def test_error_strings_without_gpu(pkg):

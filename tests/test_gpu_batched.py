@@ -76,6 +76,21 @@ def test_batch_launch_grouping_matches_oracle(pkg, batch, mode):
     plan.close()
 
 
+@pytest.mark.parametrize("from_coarse", [0, 1], ids=["steps_from_finest", "steps_from_coarsest"])
+@pytest.mark.parametrize("fuse2", [-1, 0, 256], ids=["two_launches", "two_level_all", "two_level_256"])
+def test_batch_step_alignment_and_fusion_match_oracle(pkg, batch, from_coarse, fuse2):
+    """Level steps aligned at every tile's finest level (the default: all
+    final levels, and so all two-level launches, in the last step) or at
+    the coarsest, with the last two 9/7 levels of every tile, of tiles whose
+    final level spans 256 samples both ways, or of none in one pass."""
+    _clear(batch)
+    t = pkg.BmiTuning(32, 0, 0, 0, 4, 0, fuse2, from_coarse)
+    plan = pkg.BatchPlan([d for d, _ in batch["cases"]], t)
+    plan.execute(0)
+    _check_all(batch, f"from_coarse={from_coarse} fuse2={fuse2}")
+    plan.close()
+
+
 def test_batch_persistent_queue_matches_oracle(pkg, batch):
     _clear(batch)
     plan = pkg.BatchPlan([d for d, _ in batch["cases"]], pkg.BmiTuning(32, 4, 4, 1, 0))
