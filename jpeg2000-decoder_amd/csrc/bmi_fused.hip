@@ -873,8 +873,13 @@ __device__ __attribute__((always_inline)) inline void thumb_item(const LevelJob 
 
 // Rows of the inner level kept per wave by a two-level item: the final
 // level reads them in order except near the tile edges, where the
-// symmetric extension revisits at most four rows back.
-constexpr int kRingRows = 16;
+// symmetric extension revisits rows already read.  When a row is read, at
+// most four rows have been written since it (inclusive; an emulation of
+// this schedule over a million items of every level count, offset, height
+// and segment height finds no more), so eight leave a margin of two.  A
+// small ring keeps LDS from limiting the waves per CU (three-component
+// variants: 6 KB per wave).
+constexpr int kRingRows = 8;
 
 // Two-level work item (9/7): strip `strip`, row segment `seg` of the final
 // level of J, with the level below it synthesised in the same wave.  The
