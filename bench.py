@@ -667,7 +667,7 @@ def main() -> int:
         roofline = {"bound": "hbm", "achieved": round(k_gbs, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "traffic_source": "profiles/fused_idwt97_ict_4096x2160x3.json" if traffic else None,
-                    "kernel": "fused2_step_kernel<105> (final two 9/7 levels, ICT, 24-bit pack)",
+                    "kernel": "fused2_step_kernel<137> (final two 9/7 levels, ICT, 24-bit pack; two-strip items)",
                     "kernel_us": round(k_us, 2), "images_per_launch": P,
                     "algorithmic_bytes_per_launch": k_bytes,
                     "kernel_us_one_image": round(k_us_one, 2),

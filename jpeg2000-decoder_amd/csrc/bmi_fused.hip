@@ -126,7 +126,7 @@ struct Emitter {
     int pitch, a0, b0, ox, oy, bd, od, ls, rs, mx, check_out, fmt, nc;
     float fmx, foff;
     bool pair_fast;      // both pixels of a lane in one aligned store sequence
-    bool uniform_pairs;  // every lane stores both of its pixels or neither
+    bool uniform_pairs;  // no lane of the item has just one of its pixels in the tile
 
     __device__ inline void init(const LevelJob &__restrict__ J)
     {
@@ -169,40 +169,58 @@ struct Emitter {
         else return v;
     }
 
-    // the lanes' output masks of an item: when no lane stores just one of
-    // its two pixels (all but the odd right edge of a tile), put2 takes the
-    // paired path in every lane and masks only the stores
+    // the lanes' output masks of an item: whether any lane stores just one
+    // of its two pixels (at an odd edge of the tile), tested once per store
+    // with a uniform branch
     __device__ inline void set_lanes(bool e_ok, bool o_ok)
     {
         uniform_pairs = __builtin_amdgcn_ballot_w64(e_ok != o_ok) == 0;
     }
 
-    // put2_full applies (rows inside the level, paired lanes, no element
+    // put2_full applies (rows inside the level, paired layout, no element
     // shift, no grey check-out of a colour transform; a merge keeps its
-    // grey slot choice): the common case of an item, decided once
+    // grey slot choice): the common case of an item, decided once.  Lanes
+    // that hold one pixel of their pair inside the tile (at an odd edge)
+    // stay on the same path and narrow their stores.
     __device__ inline bool full_ok() const
     {
-        if constexpr (OUTK == K_PLANE) return pair_fast && uniform_pairs;
+        if constexpr (OUTK == K_PLANE) return pair_fast;
         else if constexpr (PL == PL_BGR8 || PL == PL_ANY)
-            return pair_fast && uniform_pairs && ls == 0 && rs == 0 && (OUTK == K_MERGE || check_out < 0);
+            return pair_fast && ls == 0 && rs == 0 && (OUTK == K_MERGE || check_out < 0);
         else return false;
     }
 
-    // put2 for items where full_ok() holds: no branch but the store mask
-    __device__ inline void put2_full(int row, int ce, const T *ve, const T *vo, bool e_ok) const
+    // both samples of a lane to the planes, or the one inside the tile
+    __device__ inline void store_plane_pair(int row, int ce, const T *ve, const T *vo, bool e_ok,
+                                            bool o_ok) const
+    {
+        const uint32_t off = ((uint32_t)(row - b0) * (uint32_t)pitch + (uint32_t)(ce - a0)) * 4u;
+        if (!uniform_pairs) {
+            if (e_ok && !o_ok) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) *at(dst[c], off) = ve[c];
+            } else if (o_ok && !e_ok) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) *at(dst[c], off + 4) = vo[c];
+            }
+        }
+        if (e_ok && o_ok) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                typedef T v2 __attribute__((ext_vector_type(2)));
+                v2 pairv;
+                pairv.x = ve[c];
+                pairv.y = vo[c];
+                *(__attribute__((address_space(1))) v2 *)at(dst[c], off) = pairv;
+            }
+        }
+    }
+
+    // put2 for items where full_ok() holds: no branch but the store masks
+    __device__ inline void put2_full(int row, int ce, const T *ve, const T *vo, bool e_ok, bool o_ok) const
     {
         if constexpr (OUTK == K_PLANE) {
-            const uint32_t off = ((uint32_t)(row - b0) * (uint32_t)pitch + (uint32_t)(ce - a0)) * 4u;
-            if (e_ok) {
-#pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    typedef T v2 __attribute__((ext_vector_type(2)));
-                    v2 pairv;
-                    pairv.x = ve[c];
-                    pairv.y = vo[c];
-                    *(__attribute__((address_space(1))) v2 *)at(dst[c], off) = pairv;
-                }
-            }
+            store_plane_pair(row, ce, ve, vo, e_ok, o_ok);
         } else if constexpr (FILT == 97 && OUTK == K_RGB && NC >= 3 && (PL == PL_BGR8 || PL == PL_ANY)) {
             // full_ok implies the 24-bit BGR layout (pair_fast) and 8-bit
             // samples with no shift: a channel is
@@ -217,22 +235,39 @@ struct Emitter {
             // w0 = B0 G0 R0 B1, w1 = G1 R1 (bytes, lowest first)
             const uint32_t w0 = u8(b1, 3, u8(r0, 2, u8(g0, 1, u8(b0, 0, 0u))));
             const uint32_t w1 = u8(r1, 1, u8(g1, 0, 0u));
-            store_pair_words(row, ce, w0, w1, e_ok);
+            store_pair_words(row, ce, w0, w1, e_ok, o_ok);
         } else if constexpr (PL == PL_BGR8 || PL == PL_ANY) {
             int R0, G0, B0, A0, R1, G1, B1, A1;
             colour<false, false>(ve, R0, G0, B0, A0);
             colour<false, false>(vo, R1, G1, B1, A1);
-            store_pair_bgr8(row, ce, R0, G0, B0, R1, G1, B1, e_ok);
+            store_pair_bgr8(row, ce, R0, G0, B0, R1, G1, B1, e_ok, o_ok);
         }
     }
 
-    // store_pair_bgr8 of the six bytes already packed: w0 = B0 G0 R0 B1,
-    // w1 = G1 R1 (lowest byte first)
-    __device__ inline void store_pair_words(int row, int ce, uint32_t w0, uint32_t w1, bool e_ok) const
+    // B0 G0 R0 B1 G1 R1 of an even output column, packed as w0 = B0 G0 R0
+    // B1, w1 = G1 R1 (lowest byte first).  With an even row start (an even
+    // pitch, or an even row of an odd one) the byte offset is even: one
+    // aligned dword and one aligned short; otherwise three shorts at odd
+    // offsets.  The parity is the row's (ox + ce is even), so uniform.  A
+    // lane with one pixel inside the tile (its odd pixel past the right
+    // edge, or its even one before an odd left edge) writes that pixel's
+    // three bytes only, as a short and a byte.
+    __device__ inline void store_pair_words(int row, int ce, uint32_t w0, uint32_t w1, bool e_ok,
+                                            bool o_ok) const
     {
         const uint32_t off = ((uint32_t)(oy + row) * (uint32_t)pitch + (uint32_t)(ox + ce)) * 3u;
+        if (!uniform_pairs) {
+            if (e_ok && !o_ok) {
+                *(__attribute__((address_space(1))) uint16_t *)at(out, off) = (uint16_t)w0;
+                *(__attribute__((address_space(1))) uint8_t *)at(out, off + 2) = (uint8_t)(w0 >> 16);
+            } else if (o_ok && !e_ok) {
+                *(__attribute__((address_space(1))) uint8_t *)at(out, off + 3) = (uint8_t)(w0 >> 24);
+                *(__attribute__((address_space(1))) uint16_t *)at(out, off + 4) = (uint16_t)w1;
+            }
+        }
+        const bool both = e_ok && o_ok;
         if (((uint32_t)(oy + row) * (uint32_t)pitch) & 1u) {
-            if (e_ok) {
+            if (both) {
                 *(__attribute__((address_space(1))) uint16_t *)at(out, off) = (uint16_t)w0;
                 *(__attribute__((address_space(1))) uint16_t *)at(out, off + 2) = (uint16_t)(w0 >> 16);
                 *(__attribute__((address_space(1))) uint16_t *)at(out, off + 4) = (uint16_t)w1;
@@ -243,7 +278,7 @@ struct Emitter {
         const uint32_t hi4 = __builtin_amdgcn_alignbit(w1, w0, 16);  // R0 B1 G1 R1
         const uint32_t dw_off = mid ? off + 2 : off;
         const uint32_t sh_off = mid ? off : off + 4;
-        if (e_ok) {
+        if (both) {
             *(__attribute__((address_space(1))) uint32_t *)at(out, dw_off) = mid ? hi4 : w0;
             *(__attribute__((address_space(1))) uint16_t *)at(out, sh_off) = (uint16_t)(mid ? w0 : w1);
         }
@@ -257,38 +292,14 @@ struct Emitter {
         b = __fadd_rn(__fadd_rn(v[0], __fmul_rn(kCbB, v[1])), foff);
     }
 
-    // B0 G0 R0 B1 G1 R1 of an even output column.  With an even row start
-    // (an even pitch, or an even row of an odd one) the byte offset is even:
-    // one aligned dword and one aligned short; otherwise three shorts at odd
-    // offsets.  The parity is the row's (ox + ce is even), so uniform.
+    // the six bytes of store_pair_words from unpacked channels
     __device__ inline void store_pair_bgr8(int row, int ce, int R0, int G0, int B0, int R1, int G1, int B1,
-                                           bool e_ok) const
+                                           bool e_ok, bool o_ok) const
     {
-        const uint32_t off = ((uint32_t)(oy + row) * (uint32_t)pitch + (uint32_t)(ox + ce)) * 3u;
-        if (((uint32_t)(oy + row) * (uint32_t)pitch) & 1u) {
-            if (e_ok) {
-                *(__attribute__((address_space(1))) uint16_t *)at(out, off) =
-                    (uint16_t)((B0 & 0xff) | ((G0 & 0xff) << 8));
-                *(__attribute__((address_space(1))) uint16_t *)at(out, off + 2) =
-                    (uint16_t)((R0 & 0xff) | ((B1 & 0xff) << 8));
-                *(__attribute__((address_space(1))) uint16_t *)at(out, off + 4) =
-                    (uint16_t)((G1 & 0xff) | ((R1 & 0xff) << 8));
-            }
-            return;
-        }
-        const bool mid = (off & 2) != 0;
-        const uint32_t lo4 = (uint32_t)(B0 & 0xff) | ((uint32_t)(G0 & 0xff) << 8) |
-                             ((uint32_t)(R0 & 0xff) << 16) | ((uint32_t)(B1 & 0xff) << 24);
-        const uint32_t hi4 = (uint32_t)(R0 & 0xff) | ((uint32_t)(B1 & 0xff) << 8) |
-                             ((uint32_t)(G1 & 0xff) << 16) | ((uint32_t)(R1 & 0xff) << 24);
-        const uint32_t sh_lo = (uint32_t)(B0 & 0xff) | ((uint32_t)(G0 & 0xff) << 8);
-        const uint32_t sh_hi = (uint32_t)(G1 & 0xff) | ((uint32_t)(R1 & 0xff) << 8);
-        const uint32_t dw_off = mid ? off + 2 : off;
-        const uint32_t sh_off = mid ? off : off + 4;
-        if (e_ok) {
-            *(__attribute__((address_space(1))) uint32_t *)at(out, dw_off) = mid ? hi4 : lo4;
-            *(__attribute__((address_space(1))) uint16_t *)at(out, sh_off) = (uint16_t)(mid ? sh_lo : sh_hi);
-        }
+        const uint32_t w0 = (uint32_t)(B0 & 0xff) | ((uint32_t)(G0 & 0xff) << 8) |
+                            ((uint32_t)(R0 & 0xff) << 16) | ((uint32_t)(B1 & 0xff) << 24);
+        const uint32_t w1 = (uint32_t)(G1 & 0xff) | ((uint32_t)(R1 & 0xff) << 8);
+        store_pair_words(row, ce, w0, w1, e_ok, o_ok);
     }
 
     // colour of one pixel (before packing); SHIFT false when the output
@@ -400,26 +411,16 @@ struct Emitter {
     // colour code and masks only its stores.
     __device__ inline void put2(int row, int ce, const T *ve, const T *vo, bool e_ok, bool o_ok) const
     {
-        if (pair_fast && uniform_pairs) {
+        if (pair_fast) {
             if constexpr (OUTK == K_PLANE) {
-                const uint32_t off = ((uint32_t)(row - b0) * (uint32_t)pitch + (uint32_t)(ce - a0)) * 4u;
-                if (e_ok) {
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) {
-                        typedef T v2 __attribute__((ext_vector_type(2)));
-                        v2 pairv;
-                        pairv.x = ve[c];
-                        pairv.y = vo[c];
-                        *(__attribute__((address_space(1))) v2 *)at(dst[c], off) = pairv;
-                    }
-                }
+                store_plane_pair(row, ce, ve, vo, e_ok, o_ok);
                 return;
             } else if constexpr (PL == PL_BGR8 || PL == PL_ANY) {
                 // 6 bytes B0 G0 R0 B1 G1 R1 (pair_fast: a 24-bit layout)
                 int R0, G0, B0, A0, R1, G1, B1, A1;
                 colour(ve, R0, G0, B0, A0);
                 colour(vo, R1, G1, B1, A1);
-                store_pair_bgr8(row, ce, R0, G0, B0, R1, G1, B1, e_ok);
+                store_pair_bgr8(row, ce, R0, G0, B0, R1, G1, B1, e_ok, o_ok);
                 return;
             }
         }
@@ -653,8 +654,8 @@ __device__ __attribute__((always_inline)) inline void process_item(
         const int ko = REV ? k + 2 : k - LAG;
         if (ko >= kout0 && ko <= kout1) {
             if (full && 2 * ko >= b0 && 2 * ko + 1 < b1) {
-                em.put2_full(2 * ko, ce, evn[0], evn[1], ce_ok);
-                em.put2_full(2 * ko + 1, ce, odd[0], odd[1], ce_ok);
+                em.put2_full(2 * ko, ce, evn[0], evn[1], ce_ok, co_ok);
+                em.put2_full(2 * ko + 1, ce, odd[0], odd[1], ce_ok, co_ok);
             } else {
                 emit_row(2 * ko, evn[0], evn[1]);
                 emit_row(2 * ko + 1, odd[0], odd[1]);
@@ -875,24 +876,35 @@ __device__ __attribute__((always_inline)) inline void thumb_item(const LevelJob 
 // level reads them in order except near the tile edges, where the
 // symmetric extension revisits rows already read.  When a row is read, at
 // most four rows have been written since it (inclusive; an emulation of
-// this schedule over a million items of every level count, offset, height
-// and segment height finds no more), so eight leave a margin of two.  A
-// small ring keeps LDS from limiting the waves per CU (three-component
-// variants: 6 KB per wave).
+// this schedule, two inner steps at a time, over a million items of every
+// level count, offset, height and segment height finds no more), so eight
+// leave a margin of four.  A small ring keeps LDS from limiting the waves
+// per CU.
 constexpr int kRingRows = 8;
 
-// Two-level work item (9/7): strip `strip`, row segment `seg` of the final
-// level of J, with the level below it synthesised in the same wave.  The
-// inner level runs the same pipeline as process_item over the 64 inner
-// columns this strip's LL band needs (32 or 33 column pairs plus the
-// lifting halo: lanes 2..35 of the wave) and writes each finished row to a
-// ring of kRingRows rows in this wave's LDS; the final level takes its LL
-// samples from that ring instead of a scratch plane.  The inner level runs
-// one step (one row pair in, two rows out) whenever the final level needs a
-// row it has not produced yet.  Every sample gets the operations of the
-// two-launch form in the same order, so the output is the same bit for bit;
-// the inner level's output never goes to HBM.  Segments run top-down only.
-template <int NC, int OUTK, int PL, int KD>
+// Strips of a two-level item (LevelJob::fuse2).  One strip of the final
+// level needs 64 columns of the inner level's output (its LL band) but the
+// inner level's pipeline, like every pipeline here, lifts 128 columns per
+// wave, so with one strip per wave the inner level keeps 28 of its 64
+// lanes idle.  A two-strip item runs two strips of the final level side by
+// side (each lane owns one column pair of each), 58 + 57 output pairs,
+// whose 119 LL columns (with the final level's lifting halo) the inner
+// level makes in one 128-column pass.
+__host__ __device__ constexpr int fused2_out_pairs(int ns) { return ns == 2 ? 115 : 60; }
+
+// Two-level work item (9/7): item `strip` (NS strips side by side), row
+// segment `seg` of the final level of J, with the level below it
+// synthesised in the same wave.  The inner level runs the same pipeline as
+// process_item over the inner columns the item's LL band needs and writes
+// each finished row to a ring of kRingRows rows in this wave's LDS; the
+// final level takes its LL samples from that ring instead of a scratch
+// plane.  The inner level runs two steps (two row pairs in, four rows out)
+// whenever the final level needs a row it has not produced yet: an even
+// number of steps leaves the pipeline's state in the registers it started
+// in, so no step ends in register copies.  Every sample gets the
+// operations of the two-launch form in the same order, so the output is
+// the same bit for bit; the inner level's output never goes to HBM.
+template <int NC, int OUTK, int PL, int KD, int NS>
 __device__ __attribute__((always_inline)) inline void process_item2(
     const LevelJob &__restrict__ J, int strip, int seg, int lane, float *__restrict__ ring)
 {
@@ -900,10 +912,13 @@ __device__ __attribute__((always_inline)) inline void process_item2(
     constexpr int SK_ILL = (KD == KD_FF || KD == KD_FS) ? SK_FLOAT : (KD == KD_SS ? SK_SIGNMAG : SK_RUNTIME);
     constexpr int SK_DET = (KD == KD_FF) ? SK_FLOAT : ((KD == KD_FS || KD == KD_SS) ? SK_SIGNMAG : SK_RUNTIME);
     using T = float;
-    constexpr int HP = 2, OWP = 60;
-    constexpr int D = 2;  // final-level pairs in flight
+    constexpr int HP = 2;
+    constexpr int OWP = fused2_out_pairs(NS);
+    constexpr int RW = 64 * NS;  // ring row: floats per component
+    constexpr int D = 2;         // final-level pairs in flight
 
-    // ---- final level: as process_item, without the LL loads
+    // ---- final level: as process_item, without the LL loads; strip h of
+    // the item holds window pairs wbase + 58 h + lane
     const int a0 = J.a0, a1 = J.a1, b0 = J.b0, b1 = J.b1;
     const int lw = J.lw, lh = J.lh, rl0 = J.rl0, rh0 = J.rh0;
     const int det_pitch = J.det_pitch;
@@ -916,11 +931,27 @@ __device__ __attribute__((always_inline)) inline void process_item2(
 #pragma unroll
         for (int j = 0; j < 4; ++j) gn[c][j] = J.gain[c][j];
     }
-    const int wpair0 = ((a0 & ~1) >> 1) + strip * OWP - HP;
-    const int ce = 2 * (wpair0 + lane), co = ce + 1;
-    const bool lane_out = (lane >= HP) && (lane < 64 - HP);
-    const int lo_idx = (reflect(ce, a0, a1) >> 1) - J.cl0;
-    const int hi_off = lw + (reflect(co, a0, a1) >> 1) - J.ch0;
+    const int wbase = ((a0 & ~1) >> 1) + strip * OWP - HP;
+    const int mbase = wbase - J.cl0;  // LL column (band-relative) of ring slot 0
+    int ce[NS], lo_idx[NS], hi_off[NS], xl[NS];
+    bool ce_ok[NS], co_ok[NS];
+#pragma unroll
+    for (int h = 0; h < NS; ++h) {
+        ce[h] = 2 * (wbase + 58 * h + lane);
+        const int co = ce[h] + 1;
+        // output lanes: 2..61 of a single strip; 2..59 and 2..58 of two
+        const int lane_end = NS == 1 ? 64 - HP : (h == 0 ? 60 : 59);
+        const bool lane_out = lane >= HP && lane < lane_end;
+        lo_idx[h] = (reflect(ce[h], a0, a1) >> 1) - J.cl0;
+        hi_off[h] = lw + (reflect(co, a0, a1) >> 1) - J.ch0;
+        ce_ok[h] = lane_out && ce[h] >= a0 && ce[h] < a1;
+        co_ok[h] = lane_out && co >= a0 && co < a1;
+        // ring slot of this lane's LL sample (its unreflected column is
+        // mbase + 58 h + lane; a reflected one of a lane that matters stays
+        // inside the window, and lanes far past the tile edge only feed
+        // halo lanes that are never stored, so their slot is clamped)
+        xl[h] = min(max(lo_idx[h] - mbase, 0), RW - 1);
+    }
     const int K0 = (b0 & ~1) >> 1;
     const int Klast = (b1 - 1) >> 1;
     const int kout0 = K0 + seg * J.seg_pairs;
@@ -941,29 +972,23 @@ __device__ __attribute__((always_inline)) inline void process_item2(
     };
     // LL row (relative to the inner level's first row) of final pair k
     auto ll_row = [&](int k) { return (reflect_row(2 * k) >> 1) - rl0; };
-    auto fetch = [&](int k, uint32_t (&r)[NC][3]) {
+    auto fetch = [&](int k, uint32_t (&r)[NS][NC][3]) {
         const int rre = reflect_row(2 * k), rro = reflect_row(2 * k + 1);
         const uint32_t rt = (uint32_t)((rre >> 1) - rl0);
         const uint32_t q = (uint32_t)((rro >> 1) - rh0 + lh);
         const uint32_t dtr = rt * (uint32_t)det_pitch, dbr = q * (uint32_t)det_pitch;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            r[c][0] = ld(detp[c], dtr + (uint32_t)hi_off);
-            r[c][1] = ld(detp[c], dbr + (uint32_t)lo_idx);
-            r[c][2] = ld(detp[c], dbr + (uint32_t)hi_off);
-        }
+        for (int h = 0; h < NS; ++h)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                r[h][c][0] = ld(detp[c], dtr + (uint32_t)hi_off[h]);
+                r[h][c][1] = ld(detp[c], dbr + (uint32_t)lo_idx[h]);
+                r[h][c][2] = ld(detp[c], dbr + (uint32_t)hi_off[h]);
+            }
     };
 
-    // ---- the LDS window: inner output columns [mbase, mbase + 64)
-    // (relative to the inner level's first column); lane j of the final
-    // level reads slot lo_idx - mbase (its unreflected column is mbase + j;
-    // a reflected one of a lane that matters stays inside the window, and
-    // lanes far past the tile edge only feed halo lanes that are never
-    // stored, so their slot is clamped)
-    const int mbase = wpair0 - J.cl0;
-    const int xl = min(max(lo_idx - mbase, 0), 63);
-
-    // ---- inner level: geometry of process_item for the window's columns
+    // ---- inner level: geometry of process_item for the ring's columns
+    // [mbase, mbase + RW) (band-relative), made by lanes 2..61
     const int ia0 = J.ia0, ia1 = J.ia1, ib0 = J.ib0, ib1 = J.ib1;
     const int ilw = J.ilw, ilh = J.ilh, irl0 = J.irl0, irh0 = J.irh0;
     const int ill_pitch = J.ill_pitch;
@@ -983,8 +1008,9 @@ __device__ __attribute__((always_inline)) inline void process_item2(
     const int ilo_idx = (reflect(ice, ia0, ia1) >> 1) - J.icl0;
     const int ihi_off = ilw + (reflect(ico, ia0, ia1) >> 1) - J.ich0;
     const int xe = ice - W0, xo = ico - W0;
-    const bool we = lane >= HP && xe >= 0 && xe < 64;
-    const bool wo = lane >= HP && xo >= 0 && xo < 64;
+    const bool ilane = lane >= HP && lane < 64 - HP;
+    const bool we = ilane && xe >= 0 && xe < RW;
+    const bool wo = ilane && xo >= 0 && xo < RW;
     const int inrefl = (ib1 - ib0) >= 8 ? 1 : 3;
     auto ireflect_row = [&](int r) {
 #pragma unroll 1
@@ -1022,9 +1048,10 @@ __device__ __attribute__((always_inline)) inline void process_item2(
 
     Emitter<FILT, NC, OUTK, PL> em;
     em.init(J);
-    const bool ce_ok = lane_out && ce >= a0 && ce < a1;
-    const bool co_ok = lane_out && co >= a0 && co < a1;
-    em.set_lanes(ce_ok, co_ok);
+    bool single = false;  // some lane has one pixel of its pair in the tile
+#pragma unroll
+    for (int h = 0; h < NS; ++h) single = single || (ce_ok[h] != co_ok[h]);
+    em.set_lanes(single, false);
     const bool full = em.full_ok();
 
     // One lifting step of a pipeline (the final level's or the inner
@@ -1067,7 +1094,7 @@ __device__ __attribute__((always_inline)) inline void process_item2(
     // read by both at about the same time and come from L2 the second time.
     auto run = [&](auto dir) {
         constexpr bool REV = decltype(dir)::value;
-        // inner level state; one pair of loads in flight
+        // inner level state; two steps' loads in flight (one buffer each)
         T t0[NC][2], t1[NC][2], t2[NC][2], t3[NC][2], t4[NC][2];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
@@ -1076,19 +1103,22 @@ __device__ __attribute__((always_inline)) inline void process_item2(
         int q = REV ? iqe : iqs;  // next inner pair to consume
         int have = rmin;          // top-down: rows [rmin, have) are in the ring
         int low = rmax + 1;       // bottom-up: rows [low, rmax] are in the ring
-        uint32_t islot[NC][4];
-        ifetch(q, islot);
+        uint32_t islot[2][NC][4];
+        ifetch(q, islot[0]);
+        ifetch(REV ? max(q - 1, iqs) : min(q + 1, iqe), islot[1]);
 
-        auto inner_step = [&]() {
+        // one inner step with pair q, whose loads are in sl; refills sl
+        // with pair q + 2 (the step after next, which uses the same buffer)
+        auto inner_step = [&](uint32_t (&sl)[NC][4]) {
             T smp[NC][4];
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                smp[c][0] = to_sample<FILT, SK_ILL>(islot[c][0], ill_float, ign[c][0]);
-                smp[c][1] = to_sample<FILT, SK_DET>(islot[c][1], det_float, ign[c][1]);
-                smp[c][2] = to_sample<FILT, SK_DET>(islot[c][2], det_float, ign[c][2]);
-                smp[c][3] = to_sample<FILT, SK_DET>(islot[c][3], det_float, ign[c][3]);
+                smp[c][0] = to_sample<FILT, SK_ILL>(sl[c][0], ill_float, ign[c][0]);
+                smp[c][1] = to_sample<FILT, SK_DET>(sl[c][1], det_float, ign[c][1]);
+                smp[c][2] = to_sample<FILT, SK_DET>(sl[c][2], det_float, ign[c][2]);
+                smp[c][3] = to_sample<FILT, SK_DET>(sl[c][3], det_float, ign[c][3]);
             }
-            ifetch(REV ? max(q - 1, iqs) : min(q + 1, iqe), islot);
+            ifetch(REV ? max(q - 2, iqs) : min(q + 2, iqe), sl);
             const int po = REV ? q + 2 : q - 2;  // pair emitted by this step
             const int r0 = 2 * po - ib0;         // its first row, relative
             const bool emit = po >= qo0 && po <= qo1;
@@ -1106,12 +1136,12 @@ __device__ __attribute__((always_inline)) inline void process_item2(
                 if (emit) {
                     // rows r0 (even) and r0 + 1 (odd) of the inner output
                     if (r0 >= 0 && r0 < inh) {
-                        float *row = ring + (((r0 & (kRingRows - 1)) * NC + c) << 6);
+                        float *row = ring + ((r0 & (kRingRows - 1)) * NC + c) * RW;
                         if (we) row[xe] = ev[0];
                         if (wo) row[xo] = ev[1];
                     }
                     if (r0 + 1 >= 0 && r0 + 1 < inh) {
-                        float *row = ring + ((((r0 + 1) & (kRingRows - 1)) * NC + c) << 6);
+                        float *row = ring + (((r0 + 1) & (kRingRows - 1)) * NC + c) * RW;
                         if (we) row[xe] = od[0];
                         if (wo) row[xo] = od[1];
                     }
@@ -1125,30 +1155,42 @@ __device__ __attribute__((always_inline)) inline void process_item2(
         };
 
         // final-level pipeline state
-        T s0[NC][2], s1[NC][2], s2[NC][2], s3[NC][2], s4[NC][2];
+        T s0[NS][NC][2], s1[NS][NC][2], s2[NS][NC][2], s3[NS][NC][2], s4[NS][NC][2];
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
+        for (int h = 0; h < NS; ++h)
 #pragma unroll
-            for (int e = 0; e < 2; ++e) s0[c][e] = s1[c][e] = s2[c][e] = s3[c][e] = s4[c][e] = 0.0f;
+            for (int c = 0; c < NC; ++c)
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
+                    s0[h][c][e] = s1[h][c][e] = s2[h][c][e] = s3[h][c][e] = s4[h][c][e] = 0.0f;
 
-        uint32_t slot[D][NC][3];
+        uint32_t slot[D][NS][NC][3];
 #pragma unroll
         for (int d = 0; d < D; ++d) fetch(REV ? max(ke - d, ks) : min(ks + d, ke), slot[d]);
         // the LL samples of the next pair, read from the ring one pair ahead
-        // (the inner level runs as far as that pair needs)
-        float llnext[NC];
+        // (the inner level runs as far as that pair needs, two steps at a
+        // time; a step past the pipeline's range emits nothing)
+        float llnext[NS][NC];
         auto read_ll = [&](int k) {
             const int rk = ll_row(min(max(k, ks), ke));
             if constexpr (REV) {
 #pragma unroll 1
-                while (low > rk && q >= iqs) inner_step();
+                while (low > rk && q >= iqs) {
+                    inner_step(islot[0]);
+                    inner_step(islot[1]);
+                }
             } else {
 #pragma unroll 1
-                while (have <= rk && q <= iqe) inner_step();
+                while (have <= rk && q <= iqe) {
+                    inner_step(islot[0]);
+                    inner_step(islot[1]);
+                }
             }
-            const float *row = ring + (((rk & (kRingRows - 1)) * NC) << 6);
+            const float *row = ring + (rk & (kRingRows - 1)) * NC * RW;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) llnext[c] = row[(c << 6) + xl];
+            for (int h = 0; h < NS; ++h)
+#pragma unroll
+                for (int c = 0; c < NC; ++c) llnext[h][c] = row[c * RW + xl[h]];
         };
         read_ll(REV ? ke : ks);
         const int rounds = (ke - ks + D) / D;
@@ -1156,38 +1198,49 @@ __device__ __attribute__((always_inline)) inline void process_item2(
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 const int k = REV ? ke - (it * D + d) : ks + it * D + d;
-                T smp[NC][4];
+                T smp[NS][NC][4];
 #pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    smp[c][0] = __fmul_rn(llnext[c], gn[c][0]);
-                    smp[c][1] = to_sample<FILT, SK_DET>(slot[d][c][0], det_float, gn[c][1]);
-                    smp[c][2] = to_sample<FILT, SK_DET>(slot[d][c][1], det_float, gn[c][2]);
-                    smp[c][3] = to_sample<FILT, SK_DET>(slot[d][c][2], det_float, gn[c][3]);
-                }
+                for (int h = 0; h < NS; ++h)
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        smp[h][c][0] = __fmul_rn(llnext[h][c], gn[c][0]);
+                        smp[h][c][1] = to_sample<FILT, SK_DET>(slot[d][h][c][0], det_float, gn[c][1]);
+                        smp[h][c][2] = to_sample<FILT, SK_DET>(slot[d][h][c][1], det_float, gn[c][2]);
+                        smp[h][c][3] = to_sample<FILT, SK_DET>(slot[d][h][c][2], det_float, gn[c][3]);
+                    }
                 fetch(REV ? max(k - D, ks) : min(k + D, ke), slot[d]);
                 read_ll(REV ? k - 1 : k + 1);
 
-                T evn[2][NC], odd[2][NC];
+                T evn[NS][2][NC], odd[NS][2][NC];
 #pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    T tl = smp[c][0], th = smp[c][1], bl = smp[c][2], bh = smp[c][3];
-                    hlift<FILT>(tl, th);
-                    hlift<FILT>(bl, bh);
-                    const T Lk[2] = {tl, th}, Hk[2] = {bl, bh};
+                for (int h = 0; h < NS; ++h)
 #pragma unroll
-                    for (int e = 0; e < 2; ++e)
-                        lift_step(dir, Lk[e], Hk[e], s0[c][e], s1[c][e], s2[c][e], s3[c][e], s4[c][e],
-                                  evn[e][c], odd[e][c]);
-                }
+                    for (int c = 0; c < NC; ++c) {
+                        T tl = smp[h][c][0], th = smp[h][c][1], bl = smp[h][c][2], bh = smp[h][c][3];
+                        hlift<FILT>(tl, th);
+                        hlift<FILT>(bl, bh);
+                        const T Lk[2] = {tl, th}, Hk[2] = {bl, bh};
+#pragma unroll
+                        for (int e = 0; e < 2; ++e)
+                            lift_step(dir, Lk[e], Hk[e], s0[h][c][e], s1[h][c][e], s2[h][c][e],
+                                      s3[h][c][e], s4[h][c][e], evn[h][e][c], odd[h][e][c]);
+                    }
                 const int ko = REV ? k + 2 : k - 2;
                 if (ko >= kout0 && ko <= kout1) {
                     if (full && 2 * ko >= b0 && 2 * ko + 1 < b1) {
-                        em.put2_full(2 * ko, ce, evn[0], evn[1], ce_ok);
-                        em.put2_full(2 * ko + 1, ce, odd[0], odd[1], ce_ok);
+#pragma unroll
+                        for (int h = 0; h < NS; ++h) {
+                            em.put2_full(2 * ko, ce[h], evn[h][0], evn[h][1], ce_ok[h], co_ok[h]);
+                            em.put2_full(2 * ko + 1, ce[h], odd[h][0], odd[h][1], ce_ok[h], co_ok[h]);
+                        }
                     } else {
-                        if (2 * ko >= b0 && 2 * ko < b1) em.put2(2 * ko, ce, evn[0], evn[1], ce_ok, co_ok);
-                        if (2 * ko + 1 >= b0 && 2 * ko + 1 < b1)
-                            em.put2(2 * ko + 1, ce, odd[0], odd[1], ce_ok, co_ok);
+#pragma unroll
+                        for (int h = 0; h < NS; ++h) {
+                            if (2 * ko >= b0 && 2 * ko < b1)
+                                em.put2(2 * ko, ce[h], evn[h][0], evn[h][1], ce_ok[h], co_ok[h]);
+                            if (2 * ko + 1 >= b0 && 2 * ko + 1 < b1)
+                                em.put2(2 * ko + 1, ce[h], odd[h][0], odd[h][1], ce_ok[h], co_ok[h]);
+                        }
                     }
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -1205,8 +1258,8 @@ __device__ __attribute__((always_inline)) inline void process_item2(
 // PL_BGR8..PL_BGRA16 (3 comps), 6 merge (2-3 comps), 7 merge (4 comps).
 // kd: coefficient encodings (9/7 planes and RGB only; KD_ANY otherwise).
 // Thumbnail jobs (lev < 0): V = 64 + (filt == 97) * 8 + sub.
-// Two-level jobs (9/7 only): V = 96 + sub * 4 + kd, kd from the inner
-// level's LL encoding and the detail bands'.
+// Two-level jobs (9/7 only): V = 96 + 32 * (strips == 2) + sub * 4 + kd,
+// kd from the inner level's LL encoding and the detail bands'.
 __host__ __device__ inline int variant_of(const LevelJob &J)
 {
     int sub;
@@ -1219,14 +1272,18 @@ __host__ __device__ inline int variant_of(const LevelJob &J)
         const bool llf = (J.fuse2 ? J.ill_kind : J.ll_kind) == SRC_FLOAT, detf = J.det_kind == SRC_FLOAT;
         kd = (llf && detf) ? KD_FF : (llf ? KD_FS : (detf ? KD_ANY : KD_SS));
     }
-    if (J.fuse2) return 96 + sub * 4 + kd;
+    if (J.fuse2) return 96 + (J.fuse2 == 2 ? 32 : 0) + sub * 4 + kd;
     return (J.filt == 97 ? 32 : 0) + sub * 4 + kd;
 }
 
 // the two-level variants instantiated, as X(V); a final level whose
-// variant is not listed keeps the two-launch form
+// variant is not listed keeps the two-launch form.  There are none for one
+// component: its inner level is a third of the work of three components'
+// but runs on the same half-idle lanes, and on the 256 mixed tiles the
+// two launches took 55 us against 72-84 us for the two-level form.
+// Two-strip kernels exist for 24-bit and 48-bit BGR.
 #define BMI_FOR_EACH_FUSED_VARIANT(X)                                                       \
-    X(96) X(97) X(98) X(99) X(100) X(104) X(105) X(106) X(107) X(109) X(113) X(120)
+    X(104) X(105) X(106) X(107) X(109) X(113) X(120) X(137) X(138) X(139) X(145)
 
 __host__ __device__ inline bool fused_variant_available(int v)
 {
@@ -1240,12 +1297,13 @@ __host__ __device__ inline bool fused_variant_available(int v)
     }
 }
 
-// lockstep components of a two-level variant (its LDS ring per wave holds
-// kRingRows rows of 64 floats per component)
+// lockstep components and strips of a two-level variant (its LDS ring
+// per wave holds kRingRows rows of 64 x strips floats per component)
 __host__ __device__ constexpr int fused_variant_nc(int v)
 {
     return (((v - 96) >> 2) & 7) <= 1 ? 1 : ((((v - 96) >> 2) & 7) == 7 ? 4 : 3);
 }
+__host__ __device__ constexpr int fused_variant_strips(int v) { return v >= 128 ? 2 : 1; }
 
 template <int V>
 __device__ __attribute__((always_inline)) inline void run_variant(const LevelJob &__restrict__ J,
@@ -1387,7 +1445,7 @@ __global__ void __launch_bounds__(256) fused_step_kernel(const LevelJob *__restr
 }
 
 // level-step mode, two-level jobs: the inner level's rows live in the
-// dynamic LDS of each wave (kRingRows x components x 64 floats)
+// dynamic LDS of each wave (kRingRows x components x 64 x strips floats)
 // (Capping it at 128 VGPRs for four waves per SIMD spills and runs about
 // 1.5x slower.)
 template <int V>
@@ -1411,13 +1469,14 @@ __global__ void __launch_bounds__(256) fused2_step_kernel(const LevelJob *__rest
     const int job = __builtin_amdgcn_readfirstlane(items[first + i].job);
     const int local = __builtin_amdgcn_readfirstlane(items[first + i].local);
     const LevelJob &__restrict__ J = jobs[job];
-    float *ring = bmi_ring_lds + wave * (kRingRows * NC * 64);
+    constexpr int NS = fused_variant_strips(V);
+    float *ring = bmi_ring_lds + wave * (kRingRows * NC * 64 * NS);
     const int strip = local % J.nstrips, seg = local / J.nstrips;
-    if constexpr (SUB == 0) process_item2<1, K_PLANE, PL_ANY, KD>(J, strip, seg, lane, ring);
-    else if constexpr (SUB == 1) process_item2<1, K_MERGE, PL_ANY, KD>(J, strip, seg, lane, ring);
-    else if constexpr (SUB >= 2 && SUB <= 5) process_item2<3, K_RGB, SUB - 2, KD>(J, strip, seg, lane, ring);
-    else if constexpr (SUB == 6) process_item2<3, K_MERGE, PL_ANY, KD>(J, strip, seg, lane, ring);
-    else process_item2<4, K_MERGE, PL_ANY, KD>(J, strip, seg, lane, ring);
+    if constexpr (SUB == 0) process_item2<1, K_PLANE, PL_ANY, KD, NS>(J, strip, seg, lane, ring);
+    else if constexpr (SUB == 1) process_item2<1, K_MERGE, PL_ANY, KD, NS>(J, strip, seg, lane, ring);
+    else if constexpr (SUB >= 2 && SUB <= 5) process_item2<3, K_RGB, SUB - 2, KD, NS>(J, strip, seg, lane, ring);
+    else if constexpr (SUB == 6) process_item2<3, K_MERGE, PL_ANY, KD, NS>(J, strip, seg, lane, ring);
+    else process_item2<4, K_MERGE, PL_ANY, KD, NS>(J, strip, seg, lane, ring);
 }
 
 // level-step mode with all kernel variants of a step in one launch (each
@@ -1880,6 +1939,16 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                 J.irh0 = floor_div2(ib0);
                 J.ill_kind = (lev - 1 == 0) ? J.det_kind : SRC_FLOAT;
                 J.ill_pitch = (lev - 1 == 0) ? t.coef_pitch : P->tile_scratch_pitch[ti];
+                // two strips per item where that kernel exists (faster for
+                // one 4096x2160x3 image, 46.5 against 47.8 us, and for the
+                // 68 three-component 9/7 tiles of the 256-tile batch, 111
+                // against 126 us, although 230-column items leave more
+                // lanes past the tile edges of narrow tiles)
+                J.fuse2 = 2;
+                if (!fused_variant_available(variant_of(J))) J.fuse2 = 1;
+                const int w2 = 2 * fused2_out_pairs(J.fuse2);
+                J.nstrips = (a1 - (a0 & ~1) + w2 - 1) / w2;
+                J.n_items = J.nstrips * J.nsegs;
             };
             auto fill_fuse = [&](LevelJob &J, int slot, int c) {
                 if (!(fused && final_level)) return;
@@ -2247,7 +2316,8 @@ static int launch_one(const BmiBatchPlan *P, const BmiBatchPlan::Launch &l, hipS
         return BMI_OK;
     }
     if (l.variant >= 96) {
-        const size_t lds = (size_t)wpg * kRingRows * fused_variant_nc(l.variant) * 64 * sizeof(float);
+        const size_t lds = (size_t)wpg * kRingRows * fused_variant_nc(l.variant) * 64 *
+                           fused_variant_strips(l.variant) * sizeof(float);
         switch (l.variant) {
 #define BMI_STEP2_CASE(V)                                                                        \
     case V:                                                                                     \

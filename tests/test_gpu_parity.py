@@ -6,7 +6,7 @@ than the 1-LSB bound on 8/12-bit samples."""
 import numpy as np
 import pytest
 
-from gpu_helpers import diff_report, run_fused, run_plain
+from gpu_helpers import diff_report, make_case, run_fused, run_plain
 
 pytestmark = pytest.mark.gpu
 
@@ -86,6 +86,50 @@ def test_packed_output_matches_oracle(pkg, workloads, orc, path, filt, fmt):
     _, got = run(pkg, planes, filt=filt, levels=L, out_format=out_format, bit_depth=bd,
                  check_out=check_out)
     assert np.array_equal(got, exp), diff_report(got, exp)
+
+
+ODD_EDGES = [
+    # (x0, w, out_x): the tile's first and last columns, and where its first
+    # column lands in the picture
+    (1, 301, 5), (3, 250, 1), (0, 301, 2), (1, 130, 3), (5, 461, 7), (1, 301, 4), (2, 99, 1),
+]
+
+
+@pytest.mark.parametrize("filt,ncomp,fmt,check_out", [("97", 3, 2, -1), ("53", 3, 2, -1),
+                                                       ("97", 1, 6, 0), ("53", 1, 6, 0)],
+                         ids=["97rgb", "53rgb", "97grey", "53grey"])
+@pytest.mark.parametrize("edge", ODD_EDGES, ids=lambda e: "x%d_w%d_ox%d" % e)
+def test_odd_edge_pixels_inside_a_larger_picture(pkg, workloads, orc, filt, ncomp, fmt, check_out,
+                                                 edge):
+    """A tile at an odd column offset or of odd width, written into a larger
+    24-bit picture at (out_x, out_y): lanes whose column pair straddles the
+    tile's left or right edge store only the pixel inside the tile, on the
+    paired store path (out_x of the same parity as the tile's first column)
+    and on the per-pixel one; every byte outside the tile stays untouched."""
+    import torch
+    x0, w, out_x = edge
+    h, L, y0, out_y = 37, 4, 1, 3
+    planes = workloads.tile_planes(w, h, L, filt, ncomp, seed=w + x0, x0=x0, y0=y0, ll53=(-60, 60),
+                                   ll97=(0.0, 40.0))
+    _, exp = orc.decode_tile(planes, levels=L, filt=filt, out_format=fmt, x0=x0, y0=y0,
+                             check_out=check_out)
+    for seg in (0, 6):  # the cost model's segments, and fixed 6-row ones (two-level included)
+        desc, keep = make_case(pkg, planes, filt=filt, levels=L, out_format=fmt, check_out=check_out,
+                               x0=x0, y0=y0)
+        ow, oh = keep["ow"], keep["oh"]
+        pw, ph = ow + out_x + 5, oh + out_y + 2
+        canvas = torch.zeros((ph, pw * 3), dtype=torch.uint8, device="cuda")
+        desc.out, desc.out_pitch_px, desc.out_x, desc.out_y = canvas.data_ptr(), pw, out_x, out_y
+        tun = pkg.BmiTuning(seg, 0, 0, 0, 0, 1 if seg else 0, 0)
+        plan = pkg.BatchPlan([desc], tun)
+        plan.execute(0)
+        torch.cuda.synchronize()
+        assert plan.error_flag() == 0
+        plan.close()
+        got = canvas.cpu().numpy()
+        want = np.zeros_like(got)
+        want[out_y:out_y + oh, 3 * out_x:3 * (out_x + ow)] = exp.reshape(oh, 3 * ow)
+        assert np.array_equal(got, want), f"seg {seg}: {diff_report(got, want)}"
 
 
 @pytest.mark.parametrize("filt", ["53", "97"])
@@ -234,6 +278,10 @@ FUSE_SHAPES = [
     (4, 4, 2, 0, 0, 0), (5, 7, 2, 1, 1, 0), (9, 6, 3, 0, 1, 2), (33, 17, 3, 1, 0, 4),
     (131, 97, 4, 3, 5, 6), (257, 131, 5, 0, 0, 0), (301, 199, 5, 1, 2, 10), (130, 257, 6, 0, 0, 38),
     (600, 40, 3, 0, 0, 0), (40, 600, 3, 0, 0, 64),
+    # items of two strips (115 column pairs): widths at and around one and
+    # two items, the second strip partly or wholly past the tile edge
+    (232, 33, 2, 0, 0, 0), (233, 64, 3, 1, 0, 0), (461, 50, 3, 3, 1, 0), (120, 90, 3, 2, 0, 0),
+    (231, 20, 2, 1, 3, 4), (470, 24, 3, 0, 0, 0),
 ]
 
 

@@ -2,7 +2,7 @@
 kRingRows (8) rows per wave (bmi_fused.hip, process_item2).  This emulates
 the row schedule of process_item2 on the CPU, rows only: the final level's
 pipeline reads its LL row one pair ahead (read_ll), the inner level runs
-one step (two rows) at a time until that row exists, odd segments run
+two steps (four rows) at a time until that row exists, odd segments run
 bottom-up.  For every item of every geometry tried, the rows written since
 the row being read (inclusive) must fit in the ring, with a margin: at most
 4 here, against 8 slots."""
@@ -60,10 +60,12 @@ def _item_reach(b0, b1, ib0, ib1, seg_pairs, seg, rev):
         if rev:
             while st["low"] > rk and st["q"] >= iqs:
                 inner_step()
+                inner_step()
             assert st["low"] <= rk
             reach = max(reach, rk - st["low"] + 1)
         else:
             while st["have"] <= rk and st["q"] <= iqe:
+                inner_step()
                 inner_step()
             assert st["have"] > rk
             reach = max(reach, st["have"] - rk)
