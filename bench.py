@@ -5,7 +5,7 @@ coefficients sit in HBM -- 5-level irreversible 9/7 inverse DWT on float32
 subbands (LL ~ N(128,40), detail Laplace(4)*2^-level), inverse ICT, level
 shift, clamp, packed 24-bit BGR -- with the fused engine (one launch per
 level step, the last two levels in one launch).  `value` is the throughput
-of a stream of such images: plans of 4 images, 2 plans in flight on their
+of a stream of such images: plans of 8 images, 2 plans in flight on their
 own streams, timed with direct launches and with hipGraph replays (the
 faster is reported); the same images decoded one at a time on one stream
 (latency) are reported beside it.  The timed images rotate over buffer sets

@@ -462,6 +462,21 @@ __device__ inline typename SampleT<FILT>::T to_sample(uint32_t w, int is_float, 
     }
 }
 
+// Converted samples fixed in registers before their load buffer is
+// refilled.  Without this the compiler sinks the conversions (the gain
+// multiplications) to the samples' first use, after the refill, keeps the
+// raw words alive by copying them out of the buffer at the loop's back
+// edge, and waits for every load in flight to do so (s_waitcnt vmcnt(0)
+// once per row pair), which undoes the prefetching.
+template <typename T, int NC, int N>
+__device__ inline void pin_samples(T (&smp)[NC][N])
+{
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int j = 0; j < N; ++j) asm volatile("" : "+v"(smp[c][j]));
+}
+
 // Degenerate levels (a single row or a single column): no prefetching.
 template <int FILT, int NC, int OUTK, int PL>
 __device__ __attribute__((always_inline)) inline void process_item_small(const LevelJob &__restrict__ J,
@@ -1118,6 +1133,7 @@ __device__ __attribute__((always_inline)) inline void process_item2(
                 smp[c][2] = to_sample<FILT, SK_DET>(sl[c][2], det_float, ign[c][2]);
                 smp[c][3] = to_sample<FILT, SK_DET>(sl[c][3], det_float, ign[c][3]);
             }
+            pin_samples(smp);
             ifetch(REV ? max(q - 2, iqs) : min(q + 2, iqe), sl);
             const int po = REV ? q + 2 : q - 2;  // pair emitted by this step
             const int r0 = 2 * po - ib0;         // its first row, relative
@@ -1208,6 +1224,8 @@ __device__ __attribute__((always_inline)) inline void process_item2(
                         smp[h][c][2] = to_sample<FILT, SK_DET>(slot[d][h][c][1], det_float, gn[c][2]);
                         smp[h][c][3] = to_sample<FILT, SK_DET>(slot[d][h][c][2], det_float, gn[c][3]);
                     }
+#pragma unroll
+                for (int h = 0; h < NS; ++h) pin_samples(smp[h]);
                 fetch(REV ? max(k - D, ks) : min(k + D, ke), slot[d]);
                 read_ll(REV ? k - 1 : k + 1);
 
