@@ -5,10 +5,11 @@ coefficients sit in HBM -- 5-level irreversible 9/7 inverse DWT on float32
 subbands (LL ~ N(128,40), detail Laplace(4)*2^-level), inverse ICT, level
 shift, clamp, packed 24-bit BGR -- with the fused engine (one launch per
 level step, the last two levels in one launch).  `value` is the throughput
-of a stream of such images: plans of 8 images, 2 plans in flight on their
-own streams, timed with direct launches and with hipGraph replays (the
-faster is reported); the same images decoded one at a time on one stream
-(latency) are reported beside it.  The timed images rotate over buffer sets
+of a stream of such images: plans of up to 24 images (one plan holds all
+the timed images when there are no more, as with 20 steps), 2 plans in
+flight on their own streams when there are, timed with direct launches and
+with hipGraph replays (the faster is reported); the same images decoded
+one at a time on one stream (latency) are reported beside it.  The timed images rotate over buffer sets
 whose total exceeds the 256 MB last-level cache, so their data comes from
 HBM.  One timed output is compared with the oracle before anything is
 printed.  Prints one JSON line.
@@ -468,8 +469,9 @@ def main() -> int:
     ap.add_argument("--sets", type=int, default=8,
                     help="coefficient / output buffer sets the timed images rotate over "
                          "(8 x 133 MB: more than the 256 MB last-level cache)")
-    ap.add_argument("--images-per-plan", type=int, default=8,
-                    help="images per plan in the throughput (reported) form")
+    ap.add_argument("--images-per-plan", type=int, default=0,
+                    help="images per plan in the throughput (reported) form; 0: the timed "
+                         "images, at most 24 per plan")
     ap.add_argument("--batch-inflight", type=int, default=3,
                     help="batched leg: plans in flight on their own streams")
     ap.add_argument("--inflight", type=int, default=2,
@@ -494,6 +496,11 @@ def main() -> int:
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 256 MiB device copy, the reference for scaling HBM counters")
     args = ap.parse_args()
+    if args.images_per_plan <= 0:
+        # one plan holds the timed images (up to 24): a longer launch
+        # amortises its last round of waves (two-level launch per image:
+        # 37 us in plans of 8, 33 in plans of 16, 32 in plans of 24)
+        args.images_per_plan = max(1, min(24, args.steps))
     legs = set(x for x in args.legs.split(",") if x)
     for flag, leg in (("no_cpu", "cpu"), ("no_plain", "plain"), ("no_batched", "batched"),
                       ("no_serving", "serving"), ("no_t1", "t1")):
@@ -688,7 +695,7 @@ def main() -> int:
         log(f"timed output equals the oracle: {parity}")
         assert parity, "GPU output of the timed decode differs from the oracle"
         set_bytes = compulsory + plan.scratch_bytes
-        rec_single = {"images_per_plan": P, "plans_in_flight": S, "images_timed": images,
+        rec_single = {"images_per_plan": P, "plans_in_flight": min(S, nexec), "images_timed": images,
                       "us_per_image_by_launch_mode": {k: round(v[0] * 1e3, 2) for k, v in thr.items()},
                       "one_image_at_a_time_us": {k: round(v * 1e3, 2) for k, v in lat.items()},
                       "latency_us_rotating": round(lat_rot * 1e3, 2),
@@ -801,8 +808,11 @@ def main() -> int:
                                     "image, coefficients resident in HBM, fused engine (one launch "
                                     "per level step, the last two levels in one, "
                                     f"{launch_mode if mpix else '-'}); a stream of such images, "
-                                    f"{args.images_per_plan} per plan, {args.inflight} plans in "
-                                    f"flight on their own streams, rotating over {max(args.sets, args.images_per_plan * args.inflight)} "
+                                    + (f"all {args.steps} timed images in one plan"
+                                       if args.steps <= args.images_per_plan else
+                                       f"{args.images_per_plan} per plan, {args.inflight} plans in "
+                                       "flight on their own streams")
+                                    + f", rotating over {max(args.sets, args.images_per_plan * args.inflight)} "
                                     "buffer sets; ms_per_step is per image"}, **rec_single),
         "roofline": roofline,
         "cpu_baseline": cpu,
