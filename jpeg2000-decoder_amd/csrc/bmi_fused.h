@@ -53,6 +53,7 @@ struct LevelJob {
     int ia0, ia1, ib0, ib1;
     int ilw, ilh, icl0, ich0, irl0, irh0;
     int ill_kind, ill_pitch;
+    int iquirk;          // 5/3 reference bottom-edge rule applies to the inner level
     const void *ill[4];  // LL source of the inner level per slot
     float igain[4][4];   // 9/7 gains of the inner level per slot
 };

@@ -890,11 +890,12 @@ __device__ __attribute__((always_inline)) inline void thumb_item(const LevelJob 
 // Rows of the inner level kept per wave by a two-level item: the final
 // level reads them in order except near the tile edges, where the
 // symmetric extension revisits rows already read.  When a row is read, at
-// most four rows have been written since it (inclusive; an emulation of
-// this schedule, two inner steps at a time, over a million items of every
-// level count, offset, height and segment height finds no more), so eight
-// leave a margin of four.  A small ring keeps LDS from limiting the waves
-// per CU.
+// most four rows (9/7) or six (5/3, whose shorter pipeline lag lets two
+// inner steps run further ahead) have been written since it (inclusive;
+// an emulation of this schedule, two inner steps at a time, over a million
+// items of every level count, offset, height and segment height finds no
+// more), so eight are enough.  A small ring keeps LDS from limiting the
+// waves per CU.
 constexpr int kRingRows = 8;
 
 // Strips of a two-level item (LevelJob::fuse2).  One strip of the final
@@ -919,14 +920,16 @@ __host__ __device__ constexpr int fused2_out_pairs(int ns) { return ns == 2 ? 11
 // in, so no step ends in register copies.  Every sample gets the
 // operations of the two-launch form in the same order, so the output is
 // the same bit for bit; the inner level's output never goes to HBM.
-template <int NC, int OUTK, int PL, int KD, int NS>
+template <int FILT, int NC, int OUTK, int PL, int KD, int NS>
 __device__ __attribute__((always_inline)) inline void process_item2(
-    const LevelJob &__restrict__ J, int strip, int seg, int lane, float *__restrict__ ring)
+    const LevelJob &__restrict__ J, int strip, int seg, int lane, float *__restrict__ ring_f)
 {
-    constexpr int FILT = 97;
     constexpr int SK_ILL = (KD == KD_FF || KD == KD_FS) ? SK_FLOAT : (KD == KD_SS ? SK_SIGNMAG : SK_RUNTIME);
     constexpr int SK_DET = (KD == KD_FF) ? SK_FLOAT : ((KD == KD_FS || KD == KD_SS) ? SK_SIGNMAG : SK_RUNTIME);
-    using T = float;
+    using T = typename SampleT<FILT>::T;
+    T *__restrict__ ring = (T *)ring_f;
+    constexpr int LAG = (FILT == 97) ? 2 : 1;  // pair k in, pair k - LAG out
+    // the 5/3 items use the 9/7 geometry (two halo pairs a side; 5/3 needs one)
     constexpr int HP = 2;
     constexpr int OWP = fused2_out_pairs(NS);
     constexpr int RW = 64 * NS;  // ring row: floats per component
@@ -971,7 +974,12 @@ __device__ __attribute__((always_inline)) inline void process_item2(
     const int Klast = (b1 - 1) >> 1;
     const int kout0 = K0 + seg * J.seg_pairs;
     const int kout1 = min(Klast, kout0 + J.seg_pairs - 1);
-    const int ks = kout0 - 2, ke = kout1 + 2;
+    // 5/3: the pair at which the reference's bottom-edge rule applies (as
+    // in process_item), one more warm-up pair for it
+    const int nh = b1 - b0;
+    const int kq = (FILT == 53 && J.quirk) ? (K0 + (nh >> 1)) : -0x7fffffff;
+    const bool nh_even = !(nh & 1);
+    const int ks = kout0 - ((FILT == 97) ? 2 : (J.quirk ? 2 : 1)), ke = kout1 + LAG;
     const int nrefl = (b1 - b0) >= 8 ? 1 : 3;
     auto reflect_row = [&](int r) {
 #pragma unroll 1
@@ -1059,7 +1067,10 @@ __device__ __attribute__((always_inline)) inline void process_item2(
         rmax = max(rmax, r);
     }
     const int qo0 = (ib0 + rmin) >> 1, qo1 = (ib0 + rmax) >> 1;  // inner pairs to emit
-    const int iqs = qo0 - 2, iqe = qo1 + 2;  // inner pipeline range, warm-up included
+    // inner pipeline range, warm-up included
+    const int iqs = qo0 - ((FILT == 97) ? 2 : (J.iquirk ? 2 : 1)), iqe = qo1 + LAG;
+    const int ikq = (FILT == 53 && J.iquirk) ? (((ib0 & ~1) >> 1) + (inh >> 1)) : -0x7fffffff;
+    const bool inh_even = !(inh & 1);
 
     Emitter<FILT, NC, OUTK, PL> em;
     em.init(J);
@@ -1075,6 +1086,28 @@ __device__ __attribute__((always_inline)) inline void process_item2(
     // top-down:  p0 = H_{k-1}, p1 = e3_{k-1}, p2 = o4_{k-2}, p3 = e5_{k-2}
     // bottom-up: p0 = L_{k+1}, p1 = H_{k+1}, p2 = e3_{k+2}, p3 = o4_{k+2},
     //            p4 = e5_{k+3}
+    // 5/3, top-down only: pair k in, pair k - 1 out (process_item's 5/3
+    // pipeline); p0 = H_{k-1}, p1 = e1_{k-1}, p2 = L_{k-1}, p3 = H_{k-2};
+    // at the bottom-edge pair (at_q) the reference's VDWT_R53_LINES tail
+    auto lift53 = [](int Lk, int Hk, int &p0, int &p1, int &p2, int &p3, int &ev, int &od, bool at_q,
+                     bool h_even) {
+        int e1 = Lk - ((p0 + Hk + 2) >> 2);
+        int o2 = p0 + ((p1 + e1) >> 1);
+        if (at_q) {
+            if (h_even) {
+                o2 = Hk + p1;
+            } else {
+                e1 = p2 - ((p3 + p3 + 2) >> 2);
+                o2 = p3 + ((p1 + e1) >> 1);
+            }
+        }
+        ev = p1;
+        od = o2;
+        p3 = p0;
+        p0 = Hk;
+        p2 = Lk;
+        p1 = e1;
+    };
     auto lift_step = [](auto dir, float Lk, float Hk, float &p0, float &p1, float &p2, float &p3,
                         float &p4, float &ev, float &od) {
         constexpr bool REV = decltype(dir)::value;
@@ -1135,7 +1168,7 @@ __device__ __attribute__((always_inline)) inline void process_item2(
             }
             pin_samples(smp);
             ifetch(REV ? max(q - 2, iqs) : min(q + 2, iqe), sl);
-            const int po = REV ? q + 2 : q - 2;  // pair emitted by this step
+            const int po = REV ? q + 2 : q - LAG;  // pair emitted by this step
             const int r0 = 2 * po - ib0;         // its first row, relative
             const bool emit = po >= qo0 && po <= qo1;
 #pragma unroll
@@ -1146,18 +1179,23 @@ __device__ __attribute__((always_inline)) inline void process_item2(
                 const T Lk[2] = {tl, th}, Hk[2] = {bl, bh};
                 T ev[2], od[2];
 #pragma unroll
-                for (int e = 0; e < 2; ++e)
-                    lift_step(dir, Lk[e], Hk[e], t0[c][e], t1[c][e], t2[c][e], t3[c][e], t4[c][e],
-                              ev[e], od[e]);
+                for (int e = 0; e < 2; ++e) {
+                    if constexpr (FILT == 97)
+                        lift_step(dir, Lk[e], Hk[e], t0[c][e], t1[c][e], t2[c][e], t3[c][e], t4[c][e],
+                                  ev[e], od[e]);
+                    else
+                        lift53(Lk[e], Hk[e], t0[c][e], t1[c][e], t2[c][e], t3[c][e], ev[e], od[e], q == ikq,
+                               inh_even);
+                }
                 if (emit) {
                     // rows r0 (even) and r0 + 1 (odd) of the inner output
                     if (r0 >= 0 && r0 < inh) {
-                        float *row = ring + ((r0 & (kRingRows - 1)) * NC + c) * RW;
+                        T *row = ring + ((r0 & (kRingRows - 1)) * NC + c) * RW;
                         if (we) row[xe] = ev[0];
                         if (wo) row[xo] = ev[1];
                     }
                     if (r0 + 1 >= 0 && r0 + 1 < inh) {
-                        float *row = ring + (((r0 + 1) & (kRingRows - 1)) * NC + c) * RW;
+                        T *row = ring + (((r0 + 1) & (kRingRows - 1)) * NC + c) * RW;
                         if (we) row[xe] = od[0];
                         if (wo) row[xo] = od[1];
                     }
@@ -1186,7 +1224,7 @@ __device__ __attribute__((always_inline)) inline void process_item2(
         // the LL samples of the next pair, read from the ring one pair ahead
         // (the inner level runs as far as that pair needs, two steps at a
         // time; a step past the pipeline's range emits nothing)
-        float llnext[NS][NC];
+        T llnext[NS][NC];
         auto read_ll = [&](int k) {
             const int rk = ll_row(min(max(k, ks), ke));
             if constexpr (REV) {
@@ -1202,7 +1240,7 @@ __device__ __attribute__((always_inline)) inline void process_item2(
                     inner_step(islot[1]);
                 }
             }
-            const float *row = ring + (rk & (kRingRows - 1)) * NC * RW;
+            const T *row = ring + (rk & (kRingRows - 1)) * NC * RW;
 #pragma unroll
             for (int h = 0; h < NS; ++h)
 #pragma unroll
@@ -1219,7 +1257,8 @@ __device__ __attribute__((always_inline)) inline void process_item2(
                 for (int h = 0; h < NS; ++h)
 #pragma unroll
                     for (int c = 0; c < NC; ++c) {
-                        smp[h][c][0] = __fmul_rn(llnext[h][c], gn[c][0]);
+                        if constexpr (FILT == 97) smp[h][c][0] = __fmul_rn(llnext[h][c], gn[c][0]);
+                        else smp[h][c][0] = llnext[h][c];
                         smp[h][c][1] = to_sample<FILT, SK_DET>(slot[d][h][c][0], det_float, gn[c][1]);
                         smp[h][c][2] = to_sample<FILT, SK_DET>(slot[d][h][c][1], det_float, gn[c][2]);
                         smp[h][c][3] = to_sample<FILT, SK_DET>(slot[d][h][c][2], det_float, gn[c][3]);
@@ -1239,11 +1278,16 @@ __device__ __attribute__((always_inline)) inline void process_item2(
                         hlift<FILT>(bl, bh);
                         const T Lk[2] = {tl, th}, Hk[2] = {bl, bh};
 #pragma unroll
-                        for (int e = 0; e < 2; ++e)
-                            lift_step(dir, Lk[e], Hk[e], s0[h][c][e], s1[h][c][e], s2[h][c][e],
-                                      s3[h][c][e], s4[h][c][e], evn[h][e][c], odd[h][e][c]);
+                        for (int e = 0; e < 2; ++e) {
+                            if constexpr (FILT == 97)
+                                lift_step(dir, Lk[e], Hk[e], s0[h][c][e], s1[h][c][e], s2[h][c][e],
+                                          s3[h][c][e], s4[h][c][e], evn[h][e][c], odd[h][e][c]);
+                            else
+                                lift53(Lk[e], Hk[e], s0[h][c][e], s1[h][c][e], s2[h][c][e], s3[h][c][e],
+                                       evn[h][e][c], odd[h][e][c], k == kq, nh_even);
+                        }
                     }
-                const int ko = REV ? k + 2 : k - 2;
+                const int ko = REV ? k + 2 : k - LAG;
                 if (ko >= kout0 && ko <= kout1) {
                     if (full && 2 * ko >= b0 && 2 * ko + 1 < b1) {
 #pragma unroll
@@ -1265,10 +1309,15 @@ __device__ __attribute__((always_inline)) inline void process_item2(
             }
         }
     };
-    if (seg & 1)
-        run(std::true_type{});
-    else
+    // 9/7 odd segments run bottom-up (5/3 items always top-down)
+    if constexpr (FILT == 97) {
+        if (seg & 1)
+            run(std::true_type{});
+        else
+            run(std::false_type{});
+    } else {
         run(std::false_type{});
+    }
 }
 
 // Kernel variants: V = (filt == 97) * 32 + sub * 4 + kd.
@@ -1276,8 +1325,8 @@ __device__ __attribute__((always_inline)) inline void process_item2(
 // PL_BGR8..PL_BGRA16 (3 comps), 6 merge (2-3 comps), 7 merge (4 comps).
 // kd: coefficient encodings (9/7 planes and RGB only; KD_ANY otherwise).
 // Thumbnail jobs (lev < 0): V = 64 + (filt == 97) * 8 + sub.
-// Two-level jobs (9/7 only): V = 96 + 32 * (strips == 2) + sub * 4 + kd,
-// kd from the inner level's LL encoding and the detail bands'.
+// Two-level jobs: V = 96 + 64 * (filt == 53) + 32 * (strips == 2) + sub * 4
+// + kd, kd from the inner level's LL encoding and the detail bands' (9/7).
 __host__ __device__ inline int variant_of(const LevelJob &J)
 {
     int sub;
@@ -1290,7 +1339,7 @@ __host__ __device__ inline int variant_of(const LevelJob &J)
         const bool llf = (J.fuse2 ? J.ill_kind : J.ll_kind) == SRC_FLOAT, detf = J.det_kind == SRC_FLOAT;
         kd = (llf && detf) ? KD_FF : (llf ? KD_FS : (detf ? KD_ANY : KD_SS));
     }
-    if (J.fuse2) return 96 + (J.fuse2 == 2 ? 32 : 0) + sub * 4 + kd;
+    if (J.fuse2) return 96 + (J.filt == 53 ? 64 : 0) + (J.fuse2 == 2 ? 32 : 0) + sub * 4 + kd;
     return (J.filt == 97 ? 32 : 0) + sub * 4 + kd;
 }
 
@@ -1299,9 +1348,11 @@ __host__ __device__ inline int variant_of(const LevelJob &J)
 // component: its inner level is a third of the work of three components'
 // but runs on the same half-idle lanes, and on the 256 mixed tiles the
 // two launches took 55 us against 72-84 us for the two-level form.
-// Two-strip kernels exist for 24-bit and 48-bit BGR.
+// Two-strip kernels exist for 24-bit and 48-bit BGR; 5/3 two-level
+// kernels for those two outputs (168, 176; two strips 200, 208).
 #define BMI_FOR_EACH_FUSED_VARIANT(X)                                                       \
-    X(104) X(105) X(106) X(107) X(109) X(113) X(120) X(137) X(138) X(139) X(145)
+    X(104) X(105) X(106) X(107) X(109) X(113) X(120) X(137) X(138) X(139) X(145)             \
+    X(168) X(176) X(200) X(208)
 
 __host__ __device__ inline bool fused_variant_available(int v)
 {
@@ -1321,7 +1372,8 @@ __host__ __device__ constexpr int fused_variant_nc(int v)
 {
     return (((v - 96) >> 2) & 7) <= 1 ? 1 : ((((v - 96) >> 2) & 7) == 7 ? 4 : 3);
 }
-__host__ __device__ constexpr int fused_variant_strips(int v) { return v >= 128 ? 2 : 1; }
+__host__ __device__ constexpr int fused_variant_strips(int v) { return ((v - 96) & 32) ? 2 : 1; }
+__host__ __device__ constexpr int fused_variant_filt(int v) { return ((v - 96) & 64) ? 53 : 97; }
 
 template <int V>
 __device__ __attribute__((always_inline)) inline void run_variant(const LevelJob &__restrict__ J,
@@ -1488,13 +1540,14 @@ __global__ void __launch_bounds__(256) fused2_step_kernel(const LevelJob *__rest
     const int local = __builtin_amdgcn_readfirstlane(items[first + i].local);
     const LevelJob &__restrict__ J = jobs[job];
     constexpr int NS = fused_variant_strips(V);
+    constexpr int F = fused_variant_filt(V);
     float *ring = bmi_ring_lds + wave * (kRingRows * NC * 64 * NS);
     const int strip = local % J.nstrips, seg = local / J.nstrips;
-    if constexpr (SUB == 0) process_item2<1, K_PLANE, PL_ANY, KD, NS>(J, strip, seg, lane, ring);
-    else if constexpr (SUB == 1) process_item2<1, K_MERGE, PL_ANY, KD, NS>(J, strip, seg, lane, ring);
-    else if constexpr (SUB >= 2 && SUB <= 5) process_item2<3, K_RGB, SUB - 2, KD, NS>(J, strip, seg, lane, ring);
-    else if constexpr (SUB == 6) process_item2<3, K_MERGE, PL_ANY, KD, NS>(J, strip, seg, lane, ring);
-    else process_item2<4, K_MERGE, PL_ANY, KD, NS>(J, strip, seg, lane, ring);
+    if constexpr (SUB == 0) process_item2<F, 1, K_PLANE, PL_ANY, KD, NS>(J, strip, seg, lane, ring);
+    else if constexpr (SUB == 1) process_item2<F, 1, K_MERGE, PL_ANY, KD, NS>(J, strip, seg, lane, ring);
+    else if constexpr (SUB >= 2 && SUB <= 5) process_item2<F, 3, K_RGB, SUB - 2, KD, NS>(J, strip, seg, lane, ring);
+    else if constexpr (SUB == 6) process_item2<F, 3, K_MERGE, PL_ANY, KD, NS>(J, strip, seg, lane, ring);
+    else process_item2<F, 4, K_MERGE, PL_ANY, KD, NS>(J, strip, seg, lane, ring);
 }
 
 // level-step mode with all kernel variants of a step in one launch (each
@@ -1677,7 +1730,6 @@ int bmi_apply_ignore(const BmiTileDesc &in, BmiTileDesc *out)
 // exists for its final level's variant.
 static bool tile_fuses(const BmiTileDesc &t, int min_side)
 {
-    if (t.method != Ckernels_W9X7) return false;
     const int L = t.levels;
     const int Ls = (t.top_level < 0) ? L : t.top_level;
     if (Ls < 2) return false;
@@ -1693,14 +1745,14 @@ static bool tile_fuses(const BmiTileDesc &t, int min_side)
     needed_comps(t, ok, comps, &n);
     LevelJob J;
     memset(&J, 0, sizeof(J));
-    J.filt = 97;
+    J.filt = (t.method == Ckernels_W9X7) ? 97 : 53;
     J.lev = Ls - 1;
     J.out_kind = ok;
     J.fmt = t.out_format;
     J.nc = (ok == OUT_PLANE) ? 1 : n;
     J.fuse2 = 1;
-    J.det_kind = t.coef_kind == BMI_COEF_FLOAT ? SRC_FLOAT : SRC_SIGNMAG;
-    J.ill_kind = (Ls - 2 == 0) ? J.det_kind : SRC_FLOAT;
+    J.det_kind = J.filt == 53 ? SRC_INT : (t.coef_kind == BMI_COEF_FLOAT ? SRC_FLOAT : SRC_SIGNMAG);
+    J.ill_kind = (Ls - 2 == 0) ? J.det_kind : (J.filt == 97 ? SRC_FLOAT : SRC_INT);
     return fused_variant_available(variant_of(J));
 }
 
@@ -1955,8 +2007,13 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                 J.ich0 = floor_div2(ia0);
                 J.irl0 = ceil_div2(ib0);
                 J.irh0 = floor_div2(ib0);
-                J.ill_kind = (lev - 1 == 0) ? J.det_kind : SRC_FLOAT;
+                J.ill_kind = (lev - 1 == 0) ? J.det_kind : (filt == 97 ? SRC_FLOAT : SRC_INT);
                 J.ill_pitch = (lev - 1 == 0) ? t.coef_pitch : P->tile_scratch_pitch[ti];
+                // the inner level's own 5/3 bottom-edge rule (as base.quirk)
+                J.iquirk = (filt == 53 && t.r53_mode == BMI_R53_REFERENCE && !(ib0 & 1) &&
+                            (ib1 - ib0) >= 2)
+                               ? 1
+                               : 0;
                 // two strips per item where that kernel exists (faster for
                 // one 4096x2160x3 image, 46.5 against 47.8 us, and for the
                 // 68 three-component 9/7 tiles of the 256-tile batch, 111
@@ -1974,10 +2031,14 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                 J.ll[slot] = nullptr;  // the final level's LL comes from LDS
                 J.ill[slot] = (il == 0) ? t.coef[c] : scratch_plane(c, (il - 1) & 1);
                 const float *st = t.steps[c];
-                J.igain[slot][0] = il == 0 ? (st[0] * kLoGain) * kLoGain : kLoGain * kLoGain;
-                J.igain[slot][1] = (st[3 * il + 1] * kHiGain) * kLoGain;
-                J.igain[slot][2] = (st[3 * il + 2] * kLoGain) * kHiGain;
-                J.igain[slot][3] = (st[3 * il + 3] * kHiGain) * kHiGain;
+                if (filt == 97) {
+                    J.igain[slot][0] = il == 0 ? (st[0] * kLoGain) * kLoGain : kLoGain * kLoGain;
+                    J.igain[slot][1] = (st[3 * il + 1] * kHiGain) * kLoGain;
+                    J.igain[slot][2] = (st[3 * il + 2] * kLoGain) * kHiGain;
+                    J.igain[slot][3] = (st[3 * il + 3] * kHiGain) * kHiGain;
+                } else {
+                    for (int g = 0; g < 4; ++g) J.igain[slot][g] = 1.0f;
+                }
             };
             auto push_job = [&](const LevelJob &J, int key_comp) {
                 const int jid = (int)P->jobs.size();

@@ -286,6 +286,36 @@ FUSE_SHAPES = [
 
 
 @pytest.mark.parametrize("shape", FUSE_SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("mode", ["reference", "textbook"])
+@pytest.mark.parametrize("fmt,bd", [(2, 8), (1, 12)], ids=["bgr24", "bgr48_12bit"])
+def test_two_level_53_final_step_matches_oracle(pkg, workloads, orc, shape, mode, fmt, bd):
+    """The final two 5/3 levels in one pass: 24-bit and 48-bit (12-bit
+    samples) RGB equal the oracle and the two-launch form bit for bit, in
+    both synthesis flavours (the reference's bottom-edge rule applies to
+    each of the two levels on its own), at odd offsets, small levels and
+    fixed segment heights; the fused plan has one launch fewer."""
+    from gpu_helpers import make_case
+    w, h, L, x0, y0, seg = shape
+    r53 = orc.R53_REFERENCE if mode == "reference" else orc.R53_TEXTBOOK
+    planes = workloads.tile_planes(w, h, L, "53", 3, seed=3 * w + h, x0=x0, y0=y0, ll53=(-60, 60),
+                                   lap53=6.0 * (1 << (bd - 8)))
+    _, exp = orc.decode_tile(planes, levels=L, filt="53", out_format=fmt, bit_depth=bd, x0=x0, y0=y0,
+                             r53_mode=r53)
+    on = pkg.BmiTuning(seg, 0, 0, 0, 0, 1 if seg else 0, 0)
+    off = pkg.BmiTuning(seg, 0, 0, 0, 0, 1 if seg else 0, -1)
+    for name, t in (("fused", on), ("two launches", off)):
+        _, got = run_fused(pkg, planes, tuning=t, filt="53", levels=L, out_format=fmt, bit_depth=bd,
+                           x0=x0, y0=y0, r53_mode=r53)
+        assert np.array_equal(got, exp), f"{name}: {diff_report(got, exp)}"
+    d, _keep = make_case(pkg, planes, filt="53", levels=L, out_format=fmt, bit_depth=bd, x0=x0, y0=y0,
+                         r53_mode=r53)
+    p_on, p_off = pkg.BatchPlan([d], on), pkg.BatchPlan([d], off)
+    assert p_on.num_launches == p_off.num_launches - 1
+    p_on.close()
+    p_off.close()
+
+
+@pytest.mark.parametrize("shape", FUSE_SHAPES, ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("kind", ["float", "signmag"])
 def test_two_level_final_step_matches_oracle(pkg, workloads, orc, shape, kind):
     """The final two 9/7 levels synthesised in one pass (the lower one in an

@@ -2,17 +2,20 @@
 kRingRows (8) rows per wave (bmi_fused.hip, process_item2).  This emulates
 the row schedule of process_item2 on the CPU, rows only: the final level's
 pipeline reads its LL row one pair ahead (read_ll), the inner level runs
-two steps (four rows) at a time until that row exists, odd segments run
-bottom-up.  For every item of every geometry tried, the rows written since
+two steps (four rows) at a time until that row exists, odd 9/7 segments
+run bottom-up; 5/3 items run top-down with a pipeline lag of one pair
+(two warm-up pairs where the reference's bottom-edge rule applies).  For every item of every geometry tried, the rows written since
 the row being read (inclusive) must fit in the ring, with a margin: at most
-4 here, against 8 slots."""
+4 here for 9/7 and 6 for 5/3 (its shorter lag lets two inner steps run
+further ahead), against 8 slots (a wider sweep of heights up to 2160 and
+segment heights up to 64 finds the same maxima)."""
 import pathlib
 import re
 
 import pytest
 
 RING_ROWS = 8
-MAX_SEEN = 4
+MAX_SEEN = {"97": 4, "53": 6}
 
 
 def _level_span(t0, n, L, lev):
@@ -20,12 +23,12 @@ def _level_span(t0, n, L, lev):
     return -((-t0) >> s), -((-(t0 + n)) >> s)
 
 
-def _item_reach(b0, b1, ib0, ib1, seg_pairs, seg, rev):
+def _item_reach(b0, b1, ib0, ib1, seg_pairs, seg, rev, lag=2, warm=2, iwarm=2):
     """Largest number of ring rows a read of this item depends on."""
     K0, Klast = (b0 & ~1) >> 1, (b1 - 1) >> 1
     kout0 = K0 + seg * seg_pairs
     kout1 = min(Klast, kout0 + seg_pairs - 1)
-    ks, ke = kout0 - 2, kout1 + 2
+    ks, ke = kout0 - warm, kout1 + lag
     nrefl = 1 if (b1 - b0) >= 8 else 3
 
     def reflect_row(r):
@@ -42,12 +45,12 @@ def _item_reach(b0, b1, ib0, ib1, seg_pairs, seg, rev):
     rows = [ll_row(k) for k in range(ks, ke + 1)]
     rmin, rmax = min(rows), max(rows)
     qo0, qo1 = (ib0 + rmin) >> 1, (ib0 + rmax) >> 1
-    iqs, iqe = qo0 - 2, qo1 + 2
+    iqs, iqe = qo0 - iwarm, qo1 + lag
     st = {"q": iqe if rev else iqs, "have": rmin, "low": rmax + 1}
     reach = 0
 
     def inner_step():
-        po = st["q"] + 2 if rev else st["q"] - 2
+        po = st["q"] + 2 if rev else st["q"] - lag
         r0 = 2 * po - ib0
         if qo0 <= po <= qo1:
             st["have"] = max(st["have"], r0 + 2)
@@ -79,8 +82,9 @@ def _item_reach(b0, b1, ib0, ib1, seg_pairs, seg, rev):
     return reach
 
 
+@pytest.mark.parametrize("filt", ["97", "53"])
 @pytest.mark.parametrize("L", [2, 3, 4, 5, 6])
-def test_two_level_ring_holds_every_row_read(L):
+def test_two_level_ring_holds_every_row_read(L, filt):
     worst = 0
     for lev in range(1, L):
         for n in range(2, 97):
@@ -92,8 +96,13 @@ def test_two_level_ring_holds_every_row_read(L):
                 for sp in (1, 2, 3, 5, 16, 64):
                     K0, Klast = (b0 & ~1) >> 1, (b1 - 1) >> 1
                     for seg in range((Klast - K0 + sp) // sp):
-                        worst = max(worst, _item_reach(b0, b1, ib0, ib1, sp, seg, bool(seg & 1)))
-    assert worst <= MAX_SEEN < RING_ROWS, worst
+                        if filt == "97":
+                            worst = max(worst, _item_reach(b0, b1, ib0, ib1, sp, seg, bool(seg & 1)))
+                        else:
+                            for quirk, iquirk in ((0, 0), (1, 1), (1, 0), (0, 1)):
+                                worst = max(worst, _item_reach(b0, b1, ib0, ib1, sp, seg, False, lag=1,
+                                                               warm=1 + quirk, iwarm=1 + iquirk))
+    assert worst <= MAX_SEEN[filt] < RING_ROWS, worst
 
 
 def test_ring_size_matches_the_kernel():
