@@ -229,12 +229,14 @@ int BMI_batch_plan_error_flag(BmiBatchPlan *plan);
 /* Autotuner: build a plan for `tiles` with each candidate tuning, replay
    its graph once untimed, then time `reps` graph replays and `reps` direct
    executions with HIP events on `stream`; a candidate scores the faster
-   mean.  Returns the best in *best (its mean time per execute in
+   mean.  The six fastest are timed again with 4 x reps each, and the best
+   of that second round is returned in *best (its mean time per execute in
    *best_us).  candidates == NULL selects the built-in grid (segment rows
-   16..128 x items per CU 4..16 x 2, 4 or per-launch waves per workgroup x merged or
-   per-variant launches of mixed steps, modes 0 and 3 x two-level final steps on
-   or off).  Runs the real
-   decode: outputs of the tiles are written. */
+   16..128 x items per CU 4..16 x per-launch or 4 waves per workgroup x
+   merged or per-variant launches of mixed steps x two-level final steps
+   on, from 256 samples, or off x steps aligned at the finest or the
+   coarsest level).  Runs the real decode: outputs of the tiles are
+   written. */
 int BMI_autotune(const BmiTileDesc *tiles, int ntiles, const BmiTuning *candidates, int ncand,
                  int reps, StreamId stream, BmiTuning *best, float *best_us);
 
