@@ -184,7 +184,7 @@ typedef struct BmiTuning {
                              the smallest height (default 4) */
     int fixed_seg;       /* 1: every level step uses segments of exactly
                             seg_rows output rows (the cost model is off) */
-    int fuse2;           /* the last two 9/7 synthesis levels of a tile in
+    int fuse2;           /* the last two synthesis levels of a tile in
                             one pass (the lower one kept in LDS, never in
                             HBM): 0 default (on), -1 off, N > 0 on for tiles
                             whose final level spans at least N samples each

@@ -159,6 +159,10 @@ struct Emitter {
             // 24-bit layout (fixed by the variant, or the merge's format)
             const bool bgr8 = (PL == PL_BGR8) || (PL == PL_ANY && pixel_layout(fmt) == PL_BGR8);
             pair_fast = bgr8 && !((uintptr_t)J.out & 3) && !(ox & 1);
+            // 48-bit layout of a specialised variant: a lane's two pixels
+            // are 12 bytes at an offset that is a multiple of 4 on rows
+            // whose start is
+            if constexpr (PL == PL_BGR16) pair_fast = !((uintptr_t)J.out & 3) && !(ox & 1);
         }
     }
 
@@ -187,6 +191,8 @@ struct Emitter {
         if constexpr (OUTK == K_PLANE) return pair_fast;
         else if constexpr (PL == PL_BGR8 || PL == PL_ANY)
             return pair_fast && ls == 0 && rs == 0 && (OUTK == K_MERGE || check_out < 0);
+        else if constexpr (PL == PL_BGR16)
+            return pair_fast && (OUTK == K_MERGE || check_out < 0);
         else return false;
     }
 
@@ -241,6 +247,61 @@ struct Emitter {
             colour<false, false>(ve, R0, G0, B0, A0);
             colour<false, false>(vo, R1, G1, B1, A1);
             store_pair_bgr8(row, ce, R0, G0, B0, R1, G1, B1, e_ok, o_ok);
+        } else if constexpr (PL == PL_BGR16) {
+            int R0, G0, B0, A0, R1, G1, B1, A1;
+            colour<true, false>(ve, R0, G0, B0, A0);
+            colour<true, false>(vo, R1, G1, B1, A1);
+            store_pair_bgr16(row, ce, R0, G0, B0, R1, G1, B1, e_ok, o_ok);
+        }
+    }
+
+    // B0 G0 R0 B1 G1 R1 of an even output column as 16-bit elements (12
+    // bytes).  With an even row start the byte offset is a multiple of 4:
+    // three aligned dwords; otherwise six aligned shorts.  A lane with one
+    // pixel inside the tile writes that pixel's six bytes only (a dword and
+    // a short, or a short and a dword).
+    __device__ inline void store_pair_bgr16(int row, int ce, int R0, int G0, int B0, int R1, int G1, int B1,
+                                            bool e_ok, bool o_ok) const
+    {
+        typedef __attribute__((address_space(1))) uint32_t gu32;
+        typedef __attribute__((address_space(1))) uint16_t gu16;
+        const uint32_t off = ((uint32_t)(oy + row) * (uint32_t)pitch + (uint32_t)(ox + ce)) * 6u;
+        const uint32_t w0 = (uint32_t)(B0 & 0xffff) | ((uint32_t)G0 << 16);
+        const uint32_t w1 = (uint32_t)(R0 & 0xffff) | ((uint32_t)B1 << 16);
+        const uint32_t w2 = (uint32_t)(G1 & 0xffff) | ((uint32_t)R1 << 16);
+        const bool odd_row = (((uint32_t)(oy + row) * (uint32_t)pitch) & 1u) != 0;
+        if (!uniform_pairs) {
+            if (e_ok && !o_ok) {
+                if (odd_row) {
+                    *(gu16 *)at(out, off) = (uint16_t)w0;
+                    *(gu16 *)at(out, off + 2) = (uint16_t)(w0 >> 16);
+                } else {
+                    *(gu32 *)at(out, off) = w0;
+                }
+                *(gu16 *)at(out, off + 4) = (uint16_t)w1;
+            } else if (o_ok && !e_ok) {
+                *(gu16 *)at(out, off + 6) = (uint16_t)(w1 >> 16);
+                if (odd_row) {
+                    *(gu16 *)at(out, off + 8) = (uint16_t)w2;
+                    *(gu16 *)at(out, off + 10) = (uint16_t)(w2 >> 16);
+                } else {
+                    *(gu32 *)at(out, off + 8) = w2;
+                }
+            }
+        }
+        if (e_ok && o_ok) {
+            if (odd_row) {
+                *(gu16 *)at(out, off) = (uint16_t)w0;
+                *(gu16 *)at(out, off + 2) = (uint16_t)(w0 >> 16);
+                *(gu16 *)at(out, off + 4) = (uint16_t)w1;
+                *(gu16 *)at(out, off + 6) = (uint16_t)(w1 >> 16);
+                *(gu16 *)at(out, off + 8) = (uint16_t)w2;
+                *(gu16 *)at(out, off + 10) = (uint16_t)(w2 >> 16);
+            } else {
+                *(gu32 *)at(out, off) = w0;
+                *(gu32 *)at(out, off + 4) = w1;
+                *(gu32 *)at(out, off + 8) = w2;
+            }
         }
     }
 
@@ -421,6 +482,12 @@ struct Emitter {
                 colour(ve, R0, G0, B0, A0);
                 colour(vo, R1, G1, B1, A1);
                 store_pair_bgr8(row, ce, R0, G0, B0, R1, G1, B1, e_ok, o_ok);
+                return;
+            } else if constexpr (PL == PL_BGR16) {
+                int R0, G0, B0, A0, R1, G1, B1, A1;
+                colour(ve, R0, G0, B0, A0);
+                colour(vo, R1, G1, B1, A1);
+                store_pair_bgr16(row, ce, R0, G0, B0, R1, G1, B1, e_ok, o_ok);
                 return;
             }
         }

@@ -95,30 +95,33 @@ ODD_EDGES = [
 ]
 
 
-@pytest.mark.parametrize("filt,ncomp,fmt,check_out", [("97", 3, 2, -1), ("53", 3, 2, -1),
-                                                       ("97", 1, 6, 0), ("53", 1, 6, 0)],
-                         ids=["97rgb", "53rgb", "97grey", "53grey"])
+@pytest.mark.parametrize("filt,ncomp,fmt,check_out,bd", [
+    ("97", 3, 2, -1, 8), ("53", 3, 2, -1, 8), ("97", 1, 6, 0, 8), ("53", 1, 6, 0, 8),
+    ("97", 3, 1, -1, 12), ("53", 3, 1, -1, 12)],
+    ids=["97rgb", "53rgb", "97grey", "53grey", "97rgb48", "53rgb48"])
 @pytest.mark.parametrize("edge", ODD_EDGES, ids=lambda e: "x%d_w%d_ox%d" % e)
 def test_odd_edge_pixels_inside_a_larger_picture(pkg, workloads, orc, filt, ncomp, fmt, check_out,
-                                                 edge):
+                                                 bd, edge):
     """A tile at an odd column offset or of odd width, written into a larger
-    24-bit picture at (out_x, out_y): lanes whose column pair straddles the
-    tile's left or right edge store only the pixel inside the tile, on the
-    paired store path (out_x of the same parity as the tile's first column)
-    and on the per-pixel one; every byte outside the tile stays untouched."""
+    24-bit or 48-bit picture at (out_x, out_y): lanes whose column pair
+    straddles the tile's left or right edge store only the pixel inside the
+    tile, on the paired store path (out_x of the same parity as the tile's
+    first column; rows of both parities of the picture's odd pitch) and on
+    the per-pixel one; every byte outside the tile stays untouched."""
     import torch
     x0, w, out_x = edge
     h, L, y0, out_y = 37, 4, 1, 3
     planes = workloads.tile_planes(w, h, L, filt, ncomp, seed=w + x0, x0=x0, y0=y0, ll53=(-60, 60),
-                                   ll97=(0.0, 40.0))
-    _, exp = orc.decode_tile(planes, levels=L, filt=filt, out_format=fmt, x0=x0, y0=y0,
+                                   ll97=(0.0, 40.0), lap53=6.0 * (1 << (bd - 8)))
+    _, exp = orc.decode_tile(planes, levels=L, filt=filt, out_format=fmt, bit_depth=bd, x0=x0, y0=y0,
                              check_out=check_out)
+    pb = pkg.PIXEL_BYTES[fmt]
     for seg in (0, 6):  # the cost model's segments, and fixed 6-row ones (two-level included)
-        desc, keep = make_case(pkg, planes, filt=filt, levels=L, out_format=fmt, check_out=check_out,
-                               x0=x0, y0=y0)
+        desc, keep = make_case(pkg, planes, filt=filt, levels=L, out_format=fmt, bit_depth=bd,
+                               check_out=check_out, x0=x0, y0=y0)
         ow, oh = keep["ow"], keep["oh"]
         pw, ph = ow + out_x + 5, oh + out_y + 2
-        canvas = torch.zeros((ph, pw * 3), dtype=torch.uint8, device="cuda")
+        canvas = torch.zeros((ph, pw * pb), dtype=torch.uint8, device="cuda")
         desc.out, desc.out_pitch_px, desc.out_x, desc.out_y = canvas.data_ptr(), pw, out_x, out_y
         tun = pkg.BmiTuning(seg, 0, 0, 0, 0, 1 if seg else 0, 0)
         plan = pkg.BatchPlan([desc], tun)
@@ -128,7 +131,7 @@ def test_odd_edge_pixels_inside_a_larger_picture(pkg, workloads, orc, filt, ncom
         plan.close()
         got = canvas.cpu().numpy()
         want = np.zeros_like(got)
-        want[out_y:out_y + oh, 3 * out_x:3 * (out_x + ow)] = exp.reshape(oh, 3 * ow)
+        want[out_y:out_y + oh, pb * out_x:pb * (out_x + ow)] = exp.reshape(oh, pb * ow)
         assert np.array_equal(got, want), f"seg {seg}: {diff_report(got, want)}"
 
 
