@@ -643,25 +643,6 @@ __device__ __attribute__((always_inline)) inline void process_item(
         em.put2(row, ce, ve, vo, ce_ok, co_ok);
     };
 
-    // 9/7 segments alternate direction: odd segments run bottom-up.  The
-    // halo rows two vertically neighbouring segments both read are then
-    // read at the same time by both (at the end of each, or at the start
-    // of each), so the second read is served by L2 instead of HBM.  Every
-    // lifting step keeps its operands and their order, so the results are
-    // the same bit for bit in either direction.
-    const bool rev = (FILT == 97) && (seg & 1);
-
-    // vertical pipeline state per component and column (0 = ce, 1 = co)
-    // 9/7 top-down:  s0 = H_{k-1}, s1 = e3_{k-1}, s2 = o4_{k-2}, s3 = e5_{k-2}
-    // 9/7 bottom-up: s0 = L_{k+1}, s1 = H_{k+1}, s2 = e3_{k+2}, s3 = o4_{k+2},
-    //                s4 = e5_{k+3}
-    // 5/3: s0 = H_{k-1}, s1 = e1_{k-1}, s2 = L_{k-1}, s3 = H_{k-2}
-    T s0[NC][2], s1[NC][2], s2[NC][2], s3[NC][2], s4[NC][2];
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int e = 0; e < 2; ++e) s0[c][e] = s1[c][e] = s2[c][e] = s3[c][e] = s4[c][e] = (T)0;
-
     const int nh = b1 - b0;
     // iteration at which the reference's VDWT_R53_LINES tail applies:
     // local pair nh/2 (bmi_cpu_dwt_decoder.cpp:3359-3378)
@@ -670,6 +651,28 @@ __device__ __attribute__((always_inline)) inline void process_item(
     constexpr int LAG = (FILT == 97) ? 2 : 1;  // pair k in, pair k-LAG out
     const int ks = kout0 - ((FILT == 97) ? 2 : (quirk ? 2 : 1));
     const int ke = kout1 + LAG;
+
+    // Segments alternate direction: odd segments run bottom-up.  The halo
+    // rows two vertically neighbouring segments both read are then read at
+    // the same time by both (at the end of each, or at the start of each),
+    // so the second read is served by L2 instead of HBM.  Every lifting
+    // step keeps its operands and their order, so the results are the same
+    // bit for bit in either direction.  A 5/3 segment whose pipeline range
+    // holds the pair of the reference's bottom-edge rule runs top-down (the
+    // rule is stated on the top-down pipeline's state).
+    const bool rev = (seg & 1) && (FILT == 97 || kq < ks || kq > ke);
+
+    // vertical pipeline state per component and column (0 = ce, 1 = co)
+    // 9/7 top-down:  s0 = H_{k-1}, s1 = e3_{k-1}, s2 = o4_{k-2}, s3 = e5_{k-2}
+    // 9/7 bottom-up: s0 = L_{k+1}, s1 = H_{k+1}, s2 = e3_{k+2}, s3 = o4_{k+2},
+    //                s4 = e5_{k+3}
+    // 5/3 top-down:  s0 = H_{k-1}, s1 = e1_{k-1}, s2 = L_{k-1}, s3 = H_{k-2}
+    // 5/3 bottom-up: s0 = L_{k+1}, s1 = H_{k+1}, s2 = e1_{k+2}
+    T s0[NC][2], s1[NC][2], s2[NC][2], s3[NC][2], s4[NC][2];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) s0[c][e] = s1[c][e] = s2[c][e] = s3[c][e] = s4[c][e] = (T)0;
 
     // one pipeline step with pair k (samples already converted)
     auto consume = [&](auto dir, int k, T (&smp)[NC][4]) {
@@ -711,6 +714,14 @@ __device__ __attribute__((always_inline)) inline void process_item(
                         s3[c][e] = o4;
                         s4[c][e] = e5;
                     }
+                } else if constexpr (REV) {
+                    // pair k in, pair k + 1 out: e1_{k+1}, then o2_{k+1}
+                    const int e1 = s0[c][e] - ((Hk[e] + s1[c][e] + 2) >> 2);
+                    evn[e][c] = e1;
+                    odd[e][c] = s1[c][e] + ((e1 + s2[c][e]) >> 1);
+                    s0[c][e] = Lk[e];
+                    s1[c][e] = Hk[e];
+                    s2[c][e] = e1;
                 } else {
                     int e1 = Lk[e] - ((s0[c][e] + Hk[e] + 2) >> 2);
                     int o2 = s0[c][e] + ((s1[c][e] + e1) >> 1);
@@ -733,7 +744,7 @@ __device__ __attribute__((always_inline)) inline void process_item(
                 }
             }
         }
-        const int ko = REV ? k + 2 : k - LAG;
+        const int ko = REV ? k + LAG : k - LAG;
         if (ko >= kout0 && ko <= kout1) {
             if (full && 2 * ko >= b0 && 2 * ko + 1 < b1) {
                 em.put2_full(2 * ko, ce, evn[0], evn[1], ce_ok, co_ok);
@@ -1153,11 +1164,22 @@ __device__ __attribute__((always_inline)) inline void process_item2(
     // top-down:  p0 = H_{k-1}, p1 = e3_{k-1}, p2 = o4_{k-2}, p3 = e5_{k-2}
     // bottom-up: p0 = L_{k+1}, p1 = H_{k+1}, p2 = e3_{k+2}, p3 = o4_{k+2},
     //            p4 = e5_{k+3}
-    // 5/3, top-down only: pair k in, pair k - 1 out (process_item's 5/3
-    // pipeline); p0 = H_{k-1}, p1 = e1_{k-1}, p2 = L_{k-1}, p3 = H_{k-2};
-    // at the bottom-edge pair (at_q) the reference's VDWT_R53_LINES tail
-    auto lift53 = [](int Lk, int Hk, int &p0, int &p1, int &p2, int &p3, int &ev, int &od, bool at_q,
-                     bool h_even) {
+    // 5/3 (process_item's 5/3 pipeline), top-down: pair k in, pair k - 1
+    // out; p0 = H_{k-1}, p1 = e1_{k-1}, p2 = L_{k-1}, p3 = H_{k-2}; at the
+    // bottom-edge pair (at_q) the reference's VDWT_R53_LINES tail.
+    // Bottom-up: pair k in, pair k + 1 out; p0 = L_{k+1}, p1 = H_{k+1},
+    // p2 = e1_{k+2} (no bottom-edge pair in its range).
+    auto lift53 = [](auto dir, int Lk, int Hk, int &p0, int &p1, int &p2, int &p3, int &ev, int &od,
+                     bool at_q, bool h_even) {
+        if constexpr (decltype(dir)::value) {
+            const int e1 = p0 - ((Hk + p1 + 2) >> 2);
+            ev = e1;
+            od = p1 + ((e1 + p2) >> 1);
+            p0 = Lk;
+            p1 = Hk;
+            p2 = e1;
+            return;
+        }
         int e1 = Lk - ((p0 + Hk + 2) >> 2);
         int o2 = p0 + ((p1 + e1) >> 1);
         if (at_q) {
@@ -1235,7 +1257,7 @@ __device__ __attribute__((always_inline)) inline void process_item2(
             }
             pin_samples(smp);
             ifetch(REV ? max(q - 2, iqs) : min(q + 2, iqe), sl);
-            const int po = REV ? q + 2 : q - LAG;  // pair emitted by this step
+            const int po = REV ? q + LAG : q - LAG;  // pair emitted by this step
             const int r0 = 2 * po - ib0;         // its first row, relative
             const bool emit = po >= qo0 && po <= qo1;
 #pragma unroll
@@ -1251,8 +1273,8 @@ __device__ __attribute__((always_inline)) inline void process_item2(
                         lift_step(dir, Lk[e], Hk[e], t0[c][e], t1[c][e], t2[c][e], t3[c][e], t4[c][e],
                                   ev[e], od[e]);
                     else
-                        lift53(Lk[e], Hk[e], t0[c][e], t1[c][e], t2[c][e], t3[c][e], ev[e], od[e], q == ikq,
-                               inh_even);
+                        lift53(dir, Lk[e], Hk[e], t0[c][e], t1[c][e], t2[c][e], t3[c][e], ev[e], od[e],
+                               q == ikq, inh_even);
                 }
                 if (emit) {
                     // rows r0 (even) and r0 + 1 (odd) of the inner output
@@ -1350,11 +1372,11 @@ __device__ __attribute__((always_inline)) inline void process_item2(
                                 lift_step(dir, Lk[e], Hk[e], s0[h][c][e], s1[h][c][e], s2[h][c][e],
                                           s3[h][c][e], s4[h][c][e], evn[h][e][c], odd[h][e][c]);
                             else
-                                lift53(Lk[e], Hk[e], s0[h][c][e], s1[h][c][e], s2[h][c][e], s3[h][c][e],
-                                       evn[h][e][c], odd[h][e][c], k == kq, nh_even);
+                                lift53(dir, Lk[e], Hk[e], s0[h][c][e], s1[h][c][e], s2[h][c][e],
+                                       s3[h][c][e], evn[h][e][c], odd[h][e][c], k == kq, nh_even);
                         }
                     }
-                const int ko = REV ? k + 2 : k - LAG;
+                const int ko = REV ? k + LAG : k - LAG;
                 if (ko >= kout0 && ko <= kout1) {
                     if (full && 2 * ko >= b0 && 2 * ko + 1 < b1) {
 #pragma unroll
@@ -1376,15 +1398,13 @@ __device__ __attribute__((always_inline)) inline void process_item2(
             }
         }
     };
-    // 9/7 odd segments run bottom-up (5/3 items always top-down)
-    if constexpr (FILT == 97) {
-        if (seg & 1)
-            run(std::true_type{});
-        else
-            run(std::false_type{});
-    } else {
+    // odd segments run bottom-up, except 5/3 ones whose final or inner
+    // pipeline range holds the pair of the reference's bottom-edge rule
+    const bool rev = (seg & 1) && (FILT == 97 || ((kq < ks || kq > ke) && (ikq < iqs || ikq > iqe)));
+    if (rev)
+        run(std::true_type{});
+    else
         run(std::false_type{});
-    }
 }
 
 // Kernel variants: V = (filt == 97) * 32 + sub * 4 + kd.
@@ -2235,9 +2255,14 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
                 const int top = std::min(max_pairs, (items_at(tuned) <= 4 * simds || any2) ? 2 * tuned : tuned);
                 int best = 1;
                 long best_cost = -1;
+                // a two-level launch is never a small coarse level: one item
+                // per SIMD is enough (the 68 three-component 9/7 tiles of
+                // the 256 mixed tiles: 112 us against 129 us at eight waves
+                // per CU)
+                const long need = any2 ? std::min(target, simds) : target;
                 for (int sp = top; sp >= 1; --sp) {
                     const long total = items_at(sp);
-                    if (total < target && sp > 1) continue;
+                    if (total < need && sp > 1) continue;
                     // one wave per SIMD leaves memory latency unhidden: it
                     // costs about as much as two (final level above in
                     // 76-row segments, 1015 items: 44 us against 31 us)

@@ -2,10 +2,12 @@
 kRingRows (8) rows per wave (bmi_fused.hip, process_item2).  This emulates
 the row schedule of process_item2 on the CPU, rows only: the final level's
 pipeline reads its LL row one pair ahead (read_ll), the inner level runs
-two steps (four rows) at a time until that row exists, odd 9/7 segments
-run bottom-up; 5/3 items run top-down with a pipeline lag of one pair
-(two warm-up pairs where the reference's bottom-edge rule applies).  For every item of every geometry tried, the rows written since
-the row being read (inclusive) must fit in the ring, with a margin: at most
+two steps (four rows) at a time until that row exists, odd segments run
+bottom-up (5/3 ones only where neither pipeline's range holds the pair of
+the reference's bottom-edge rule); 5/3 pipelines lag one pair (two warm-up
+pairs where that rule applies).  For every item of every geometry tried,
+the rows written since the row being read (inclusive) must fit in the
+ring, with a margin: at most
 4 here for 9/7 and 6 for 5/3 (its shorter lag lets two inner steps run
 further ahead), against 8 slots (a wider sweep of heights up to 2160 and
 segment heights up to 64 finds the same maxima)."""
@@ -23,8 +25,9 @@ def _level_span(t0, n, L, lev):
     return -((-t0) >> s), -((-(t0 + n)) >> s)
 
 
-def _item_reach(b0, b1, ib0, ib1, seg_pairs, seg, rev, lag=2, warm=2, iwarm=2):
-    """Largest number of ring rows a read of this item depends on."""
+def _item_reach(b0, b1, ib0, ib1, seg_pairs, seg, rev, lag=2, warm=2, iwarm=2, quirk=0, iquirk=0):
+    """Largest number of ring rows a read of this item depends on.  With
+    rev None the direction is the kernel's rule for a 5/3 item."""
     K0, Klast = (b0 & ~1) >> 1, (b1 - 1) >> 1
     kout0 = K0 + seg * seg_pairs
     kout1 = min(Klast, kout0 + seg_pairs - 1)
@@ -46,11 +49,16 @@ def _item_reach(b0, b1, ib0, ib1, seg_pairs, seg, rev, lag=2, warm=2, iwarm=2):
     rmin, rmax = min(rows), max(rows)
     qo0, qo1 = (ib0 + rmin) >> 1, (ib0 + rmax) >> 1
     iqs, iqe = qo0 - iwarm, qo1 + lag
+    if rev is None:
+        kq = K0 + ((b1 - b0) >> 1) if quirk else None
+        ikq = ((ib0 & ~1) >> 1) + ((ib1 - ib0) >> 1) if iquirk else None
+        rev = bool(seg & 1) and (kq is None or not ks <= kq <= ke) and \
+            (ikq is None or not iqs <= ikq <= iqe)
     st = {"q": iqe if rev else iqs, "have": rmin, "low": rmax + 1}
     reach = 0
 
     def inner_step():
-        po = st["q"] + 2 if rev else st["q"] - lag
+        po = st["q"] + lag if rev else st["q"] - lag
         r0 = 2 * po - ib0
         if qo0 <= po <= qo1:
             st["have"] = max(st["have"], r0 + 2)
@@ -100,8 +108,9 @@ def test_two_level_ring_holds_every_row_read(L, filt):
                             worst = max(worst, _item_reach(b0, b1, ib0, ib1, sp, seg, bool(seg & 1)))
                         else:
                             for quirk, iquirk in ((0, 0), (1, 1), (1, 0), (0, 1)):
-                                worst = max(worst, _item_reach(b0, b1, ib0, ib1, sp, seg, False, lag=1,
-                                                               warm=1 + quirk, iwarm=1 + iquirk))
+                                worst = max(worst, _item_reach(b0, b1, ib0, ib1, sp, seg, None, lag=1,
+                                                               warm=1 + quirk, iwarm=1 + iquirk,
+                                                               quirk=quirk, iquirk=iquirk))
     assert worst <= MAX_SEEN[filt] < RING_ROWS, worst
 
 
