@@ -90,6 +90,10 @@ double now_us(std::chrono::steady_clock::time_point t0)
 
 }  // namespace
 
+// initial pinned staging per stream: the job and item tables of a batch of
+// 32 requests of up to 8192x8192 pixels
+constexpr size_t kStagingBytes = size_t(4) << 20;
+
 struct BmiServer {
     BmiServerConfig cfg{};
     int device = 0;
@@ -363,8 +367,15 @@ extern "C" int BMI_server_create(BmiServer **out, const BmiServerConfig *cfg)
         }
         S->streams.push_back(s);
         S->stream_busy.push_back(false);
-        S->staging.push_back(nullptr);
-        S->staging_bytes.push_back(0);
+        // pinned staging for the plan tables, allocated here rather than on
+        // the stream's first batch (a pinned allocation takes a few hundred
+        // microseconds, which the first batches of a burst would wait for);
+        // a batch with larger tables grows it
+        void *h = nullptr;
+        const bool fused = S->cfg.mode == BMI_SERVE_FUSED;
+        if (fused && hipHostMalloc(&h, kStagingBytes, 0) != hipSuccess) h = nullptr;
+        S->staging.push_back(h);
+        S->staging_bytes.push_back(h ? kStagingBytes : 0);
     }
     S->t0 = std::chrono::steady_clock::now();
     S->thread = std::thread([S] { S->run(); });
