@@ -667,13 +667,16 @@ def main() -> int:
         k_bytes = P * k_img
         k_gbs = k_bytes / (k_us * 1e-6) / 1e9
         k_gbs_one = k_img / (k_us_one * 1e-6) / 1e9
+        # measured HBM bytes of the same launch (rocprofv3 FETCH_SIZE and
+        # WRITE_SIZE, calibrated), scaled from the profiled launch's image
+        # count to this one's
         traffic = None
         prof = ROOT / "profiles" / "fused_idwt97_ict_4096x2160x3.json"
         if prof.exists() and (W, H) == (4096, 2160):
             pj = json.loads(prof.read_text())
             for kk in pj["kernels"]:
-                if kk.get("final_level"):
-                    traffic = int(kk["read_bytes"] + kk["write_bytes"])
+                if kk.get("final_level") and kk.get("images_per_launch"):
+                    traffic = int((kk["read_bytes"] + kk["write_bytes"]) * P / kk["images_per_launch"])
         roofline = {"bound": "hbm", "achieved": round(k_gbs, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "traffic_source": "profiles/fused_idwt97_ict_4096x2160x3.json" if traffic else None,
