@@ -2049,6 +2049,12 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
             const int nrows = b1 - (b0 & ~1);
             base.seg_pairs = seg_rows / 2;
             base.nsegs = (nrows + seg_rows - 1) / seg_rows;
+            // An empty level (a tile of a few samples at an odd offset can
+            // have coarse levels of no rows or no columns) gets a job with
+            // no items: the kernels' row and column reflections assume at
+            // least one sample.  The level above then has at most one row
+            // (or column), a high-band one, and reads none of its output.
+            if (a1 <= a0 || b1 <= b0) base.nstrips = base.nsegs = 0;
             base.filt = filt;
             base.lev = lev;
             base.out_kind = ok;
@@ -2369,6 +2375,12 @@ int bmi_plan_build(BmiBatchPlan **out_plan, const BmiTileDesc *tiles, int ntiles
         }
         P->step_count[step] = (int)P->items.size() - P->step_first[step];
     }
+
+    // a step whose jobs are all empty levels has nothing to launch (a grid
+    // of no workgroups is an invalid launch)
+    P->launches.erase(std::remove_if(P->launches.begin(), P->launches.end(),
+                                     [](const BmiBatchPlan::Launch &l) { return l.count <= 0; }),
+                      P->launches.end());
 
     // Waves per workgroup of each step launch.  A launch of a specialised
     // kernel with fewer than four waves' worth of items per SIMD runs one

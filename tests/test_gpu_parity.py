@@ -318,6 +318,46 @@ def test_two_level_53_final_step_matches_oracle(pkg, workloads, orc, shape, mode
     p_off.close()
 
 
+@pytest.mark.parametrize("seg", [2, 4])
+@pytest.mark.parametrize("fuse", [0, -1], ids=["two_level", "one_level"])
+def test_53_segments_around_the_bottom_edge_rule(pkg, workloads, orc, seg, fuse):
+    """5/3 in the reference's flavour with segments of one or two row pairs:
+    odd segments run bottom-up except where a pipeline's range holds the
+    pair of the bottom-edge rule (the last segment, or the last two when
+    the last holds one pair, and the inner level's own pair).  Every height
+    of 2 to 41 rows at each row offset 0..3 equals the oracle; heights of
+    2 or 3 rows at odd offsets have an empty coarsest level (no rows),
+    which gets no work items."""
+    L, w = 3, 21
+    t = pkg.BmiTuning(seg, 0, 0, 0, 0, 1, fuse)
+    for h in range(2, 42):
+        for y0 in range(4):
+            planes = workloads.tile_planes(w, h, L, "53", 3, seed=h * 4 + y0, y0=y0, ll53=(-60, 60))
+            _, exp = orc.decode_tile(planes, levels=L, filt="53", out_format=2, y0=y0,
+                                     r53_mode=orc.R53_REFERENCE)
+            _, got = run_fused(pkg, planes, tuning=t, filt="53", levels=L, out_format=2, y0=y0,
+                               r53_mode=orc.R53_REFERENCE)
+            assert np.array_equal(got, exp), f"h {h} y0 {y0}: {diff_report(got, exp)}"
+
+
+@pytest.mark.parametrize("filt", ["53", "97"])
+def test_empty_coarse_levels(pkg, workloads, orc, filt):
+    """Tiles of 1 to 4 samples a side at offsets 0..3 with 3 levels: their
+    coarse levels can have no columns or no rows.  Such a level gets no
+    work items (its span would send the kernels' reflections outside the
+    bands); the fused and the plain path equal the oracle."""
+    L = 3
+    for w, h in ((1, 5), (2, 5), (3, 6), (5, 2), (6, 3), (2, 2), (3, 3), (4, 1), (1, 4)):
+        for x0, y0 in ((0, 0), (1, 1), (3, 2), (2, 3), (1, 3)):
+            planes = workloads.tile_planes(w, h, L, filt, 3, seed=w * 7 + h + x0, x0=x0, y0=y0,
+                                           ll53=(-60, 60), ll97=(0.0, 40.0))
+            _, exp = orc.decode_tile(planes, levels=L, filt=filt, out_format=2, x0=x0, y0=y0)
+            for run in (run_fused, run_plain):
+                _, got = run(pkg, planes, filt=filt, levels=L, out_format=2, x0=x0, y0=y0)
+                assert np.array_equal(got, exp), f"{run.__name__} {w}x{h} at ({x0},{y0}): " + \
+                    diff_report(got, exp)
+
+
 @pytest.mark.parametrize("shape", FUSE_SHAPES, ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("kind", ["float", "signmag"])
 def test_two_level_final_step_matches_oracle(pkg, workloads, orc, shape, kind):
