@@ -222,6 +222,16 @@ int BMI_batch_plan_num_launches(const BmiBatchPlan *plan);
 int BMI_batch_plan_execute_launches(BmiBatchPlan *plan, int first, int count, StreamId stream);
 int BMI_batch_plan_num_items(const BmiBatchPlan *plan);
 int BMI_batch_plan_destroy(BmiBatchPlan *plan);
+/* Host-side description of the plan BMI_batch_plan_create would build,
+   made without device memory or a GPU: up to max_jobs job records (raw
+   internal LevelJob structs of *job_bytes bytes each; pointers into the
+   plan's own memory are offsets below *device_bytes) and the item count of
+   up to max_launches launches.  No counterpart in the reference: it lets
+   tests and tools check on the CPU every address the kernels will touch. */
+int BMI_batch_plan_describe(const BmiTileDesc *tiles, int ntiles, const BmiTuning *tuning,
+                            void *jobs, int max_jobs, int *njobs, int *job_bytes,
+                            int *launch_counts, int max_launches, int *nlaunches,
+                            size_t *device_bytes);
 /* Non-zero when a persistent launch gave up waiting for a dependency
    (bounded spin) instead of hanging; synchronous. */
 int BMI_batch_plan_error_flag(BmiBatchPlan *plan);

@@ -2731,6 +2731,34 @@ extern "C" size_t BMI_batch_plan_scratch_bytes(const BmiBatchPlan *P)
 
 extern "C" int BMI_batch_plan_num_items(const BmiBatchPlan *P) { return P ? (int)P->items.size() : 0; }
 
+// Host-side description of the plan BMI_batch_plan_create would build for
+// these tiles, without device memory: its job table as raw LevelJob
+// records (pointers into the plan's own memory are offsets from 0, below
+// *device_bytes) and the item count of each launch.  For tests and tools
+// that check, on the CPU, what the kernels will address.
+extern "C" int BMI_batch_plan_describe(const BmiTileDesc *tiles, int ntiles, const BmiTuning *tuning,
+                                       void *jobs, int max_jobs, int *njobs, int *job_bytes,
+                                       int *launch_counts, int max_launches, int *nlaunches,
+                                       size_t *device_bytes)
+{
+    if (!njobs || !job_bytes || !nlaunches || !device_bytes || max_jobs < 0 || max_launches < 0 ||
+        (max_jobs > 0 && !jobs) || (max_launches > 0 && !launch_counts))
+        return bmi_fail(BMI_E_INVALID, "batch_plan_describe: bad args");
+    BmiBatchPlan *P = nullptr;
+    const int rc = bmi_plan_build(&P, tiles, ntiles, tuning);
+    if (rc) return rc;
+    *njobs = (int)P->jobs.size();
+    *job_bytes = (int)sizeof(LevelJob);
+    *nlaunches = (int)P->launches.size();
+    *device_bytes = P->device_bytes;
+    const int nj = std::min(max_jobs, (int)P->jobs.size());
+    if (nj > 0) memcpy(jobs, P->jobs.data(), sizeof(LevelJob) * (size_t)nj);
+    const int nl = std::min(max_launches, (int)P->launches.size());
+    for (int i = 0; i < nl; ++i) launch_counts[i] = P->launches[i].count;
+    delete P;
+    return BMI_OK;
+}
+
 extern "C" int BMI_batch_plan_destroy(BmiBatchPlan *P)
 {
     if (!P) return BMI_OK;
