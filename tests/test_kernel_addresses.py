@@ -232,6 +232,23 @@ def item_two_level(buf, J, strip, seg, pb):
                 buf.load32(J.det[c], dbr + ihi, what)
 
 
+def item_thumbnail(buf, J, strip, seg, pb):
+    """thumb_item (bmi_fused.hip): one LL sample per lane, colour and store."""
+    what = ("thumbnail", strip, seg)
+    for lane in range(64):
+        x = J.a0 + strip * 64 + lane
+        y = J.b0 + seg
+        if x >= J.a1:
+            continue
+        for c in range(J.nc):
+            buf.touch(J.ll[c], 4 * (seg * J.ll_pitch + (x - J.a0)), 4, what)
+        if J.out_kind in (OUT_SCRATCH, OUT_PLANE):
+            for c in range(J.nc):
+                buf.touch(J.dst[c], 4 * ((y - J.b0) * J.dst_pitch + (x - J.a0)), 4, what)
+        else:
+            pixel_store(buf, J, y, x, pb, what)
+
+
 def describe(pkg, desc, tuning):
     lib = pkg.lib()
     f = lib.BMI_batch_plan_describe
@@ -273,8 +290,8 @@ def check_tile(pkg, w, h, levels, x0, y0, filt, ncomp, fmt, r53, seg, fuse, top=
         for local in range(J.n_items):
             strip, segi = local % J.nstrips, local // J.nstrips
             if J.lev < 0:
-                continue  # thumbnail jobs (top_level 0) are not swept here
-            if J.fuse2:
+                item_thumbnail(buf, J, strip, segi, pb)
+            elif J.fuse2:
                 item_two_level(buf, J, strip, segi, pb)
             else:
                 item_one_level(buf, J, strip, segi, pb)
@@ -303,10 +320,12 @@ def test_every_kernel_address_inside_its_buffer(pkg, case):
         assert not bad, f"{ncomp} comps fmt {fmt}: {bad[:3]} ({len(bad)} accesses)"
 
 
-def test_reduced_resolution_addresses(pkg):
-    for w, h, x0, y0, top in ((9, 7, 1, 2, 1), (70, 21, 3, 1, 2), (5, 6, 2, 3, 3), (21, 70, 0, 1, 1)):
-        bad = check_tile(pkg, w, h, 4, x0, y0, 97, 3, pkg.FMT_24BITS_RGB, 0, 0, 0, top=top)
-        assert not bad, bad[:3]
+def test_reduced_resolution_and_thumbnail_addresses(pkg):
+    for w, h, x0, y0, top in ((9, 7, 1, 2, 1), (70, 21, 3, 1, 2), (5, 6, 2, 3, 3), (21, 70, 0, 1, 1),
+                              (33, 40, 1, 3, 0), (70, 21, 3, 1, 0), (17, 18, 1, 1, 0), (130, 5, 2, 0, 0)):
+        for filt, ncomp, fmt in ((97, 3, pkg.FMT_24BITS_RGB), (53, 1, pkg.FMT_NONE)):
+            bad = check_tile(pkg, w, h, 4, x0, y0, filt, ncomp, fmt, 0, 0, 0, top=top)
+            assert not bad, (w, h, x0, y0, top, bad[:3])
 
 
 def test_empty_level_plans_have_no_empty_launches(pkg):
