@@ -250,18 +250,22 @@ def item_thumbnail(buf, J, strip, seg, pb):
 
 
 def describe(pkg, desc, tuning):
+    """BMI_batch_plan_describe of one tile, or of a list of tiles."""
     lib = pkg.lib()
     f = lib.BMI_batch_plan_describe
     f.restype = ctypes.c_int
-    jobs = (LevelJob * 64)()
-    counts = (ctypes.c_int * 64)()
+    descs = desc if isinstance(desc, list) else [desc]
+    arr = (pkg.BmiTileDesc * len(descs))(*descs)
+    cap = 64 + 32 * len(descs)
+    jobs = (LevelJob * cap)()
+    counts = (ctypes.c_int * cap)()
     nj, jb, nl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     dev = ctypes.c_size_t()
-    rc = f(ctypes.byref(desc), 1, ctypes.byref(tuning), jobs, 64, ctypes.byref(nj), ctypes.byref(jb),
-           counts, 64, ctypes.byref(nl), ctypes.byref(dev))
+    rc = f(arr, len(descs), ctypes.byref(tuning), jobs, cap, ctypes.byref(nj), ctypes.byref(jb),
+           counts, cap, ctypes.byref(nl), ctypes.byref(dev))
     assert rc == 0, pkg.lib().BMI_error_string(rc)
     assert jb.value == ctypes.sizeof(LevelJob)
-    assert nj.value <= 64 and nl.value <= 64
+    assert nj.value <= cap and nl.value <= cap
     return list(jobs[:nj.value]), list(counts[:nl.value]), dev.value
 
 
@@ -338,3 +342,34 @@ def test_empty_level_plans_have_no_empty_launches(pkg):
     empty = [J for J in jobs if J.b1 <= J.b0 or J.a1 <= J.a0]
     assert empty and all(J.n_items == 0 for J in empty)
     assert counts and all(n > 0 for n in counts)
+
+
+@pytest.mark.parametrize("mode", [0, 3])
+def test_mixed_batch_addresses(pkg, workloads, mode):
+    """A plan of 24 tiles of the mixed-batch distribution (sides shrunk to
+    keep the emulation short; odd sizes, both filters, 1 or 3 components):
+    every tile's accesses stay in its own planes, output and scratch."""
+    tiles = workloads.config3_params(0)[:24]
+    descs, bufs = [], Buffers()
+    for i, t in enumerate(tiles):
+        w, h = 7 + t["w"] // 16, 5 + t["h"] // 16
+        nc = t["ncomp"]
+        coef = [(1 + 4 * i + c) << 36 for c in range(nc)]
+        out = (0x200 + i) << 36
+        fmt, co = (pkg.FMT_24BITS_RGB, -1) if nc == 3 else (pkg.FMT_24BITS_3COMPONENTS, 0)
+        descs.append(pkg.tile_desc(coef_ptrs=coef, coef_pitch=w, width=w, height=h, x0=i % 3, y0=i % 2,
+                                   levels=min(t["levels"], 4),
+                                   method=pkg.W5X3 if t["filt"] == "53" else pkg.W9X7, ncomp=nc,
+                                   out_format=fmt, check_out=co, out_ptr=out, out_pitch_px=w))
+        for c in range(nc):
+            bufs.add(coef[c], w * h * 4, f"tile{i} coef{c}")
+        ow, oh = pkg.output_size(w, h, i % 3, i % 2, min(t["levels"], 4))
+        bufs.add(out, ow * oh * 3, f"tile{i} out")
+    jobs, counts, dev = describe(pkg, descs, pkg.BmiTuning(32, 0, 0, mode, 0, 0, 0))
+    assert all(n > 0 for n in counts)
+    bufs.add(0, dev, "plan memory")
+    for J in jobs:
+        for local in range(J.n_items):
+            strip, segi = local % J.nstrips, local // J.nstrips
+            (item_two_level if J.fuse2 else item_one_level)(bufs, J, strip, segi, 3)
+    assert not bufs.bad, bufs.bad[:3]
