@@ -184,6 +184,10 @@ int or_idwt53(int32_t *plane, int pitch, int w, int h, int x0, int y0, int level
         or_level_span(x0, w, L, lev, &a0, &a1);
         or_level_span(y0, h, L, lev, &b0, &b1);
         int nw = a1 - a0, nh = b1 - b0;
+        /* an empty level (a tile of a few samples at an odd offset) has no
+           samples to transform; the reference's line loops are undefined
+           for it (its 5/3 column tail would index row -1) */
+        if (nw <= 0 || nh <= 0) continue;
         int lw = ceil2(a1) - ceil2(a0), lh = ceil2(b1) - ceil2(b0);
         int px = a0 & 1, py = b0 & 1;
         int refx = (mode == OR_R53_REFERENCE) && !px;
@@ -254,6 +258,10 @@ int or_fdwt53(const int32_t *in, int w, int h, int x0, int y0, int levels, int32
         or_level_span(x0, w, L, lev, &a0, &a1);
         or_level_span(y0, h, L, lev, &b0, &b1);
         int nw = a1 - a0, nh = b1 - b0;
+        /* an empty level (a tile of a few samples at an odd offset) has no
+           samples to transform; the reference's line loops are undefined
+           for it (its 5/3 column tail would index row -1) */
+        if (nw <= 0 || nh <= 0) continue;
         int lw = ceil2(a1) - ceil2(a0), lh = ceil2(b1) - ceil2(b0);
         int px = a0 & 1, py = b0 & 1;
         for (int c = 0; c < nw; ++c) {
@@ -369,6 +377,10 @@ int or_idwt97(uint32_t *plane, int pitch, int w, int h, int x0, int y0, int leve
         or_level_span(x0, w, L, lev, &a0, &a1);
         or_level_span(y0, h, L, lev, &b0, &b1);
         int nw = a1 - a0, nh = b1 - b0;
+        /* an empty level (a tile of a few samples at an odd offset) has no
+           samples to transform; the reference's line loops are undefined
+           for it (its 5/3 column tail would index row -1) */
+        if (nw <= 0 || nh <= 0) continue;
         int lw = ceil2(a1) - ceil2(a0), lh = ceil2(b1) - ceil2(b0);
         int px = a0 & 1, py = b0 & 1;
         float gTopLo = lev == 0 ? (steps[0] * F_LO) * F_LO : F_LO * F_LO;
@@ -426,6 +438,10 @@ int or_idwt97_f64(const uint32_t *plane, int pitch, int w, int h, int x0, int y0
         or_level_span(x0, w, L, lev, &a0, &a1);
         or_level_span(y0, h, L, lev, &b0, &b1);
         int nw = a1 - a0, nh = b1 - b0;
+        /* an empty level (a tile of a few samples at an odd offset) has no
+           samples to transform; the reference's line loops are undefined
+           for it (its 5/3 column tail would index row -1) */
+        if (nw <= 0 || nh <= 0) continue;
         int lw = ceil2(a1) - ceil2(a0), lh = ceil2(b1) - ceil2(b0);
         int px = a0 & 1, py = b0 & 1;
         double gTopLo = (lev == 0 ? (double)steps[0] : 1.0) * lo * lo;
@@ -492,6 +508,10 @@ int or_fdwt97_f64(const double *in, int w, int h, int x0, int y0, int levels, do
         or_level_span(x0, w, L, lev, &a0, &a1);
         or_level_span(y0, h, L, lev, &b0, &b1);
         int nw = a1 - a0, nh = b1 - b0;
+        /* an empty level (a tile of a few samples at an odd offset) has no
+           samples to transform; the reference's line loops are undefined
+           for it (its 5/3 column tail would index row -1) */
+        if (nw <= 0 || nh <= 0) continue;
         int lw = ceil2(a1) - ceil2(a0), lh = ceil2(b1) - ceil2(b0);
         int px = a0 & 1, py = b0 & 1;
         for (int c = 0; c < nw; ++c) {

@@ -269,14 +269,14 @@ def describe(pkg, desc, tuning):
     return list(jobs[:nj.value]), list(counts[:nl.value]), dev.value
 
 
-def check_tile(pkg, w, h, levels, x0, y0, filt, ncomp, fmt, r53, seg, fuse, top=-1):
+def check_tile(pkg, w, h, levels, x0, y0, filt, ncomp, fmt, r53, seg, fuse, top=-1, bit_depth=None):
     coef = [(c + 1) << 36 for c in range(ncomp)]
     planes = [(0xA + c) << 36 for c in range(ncomp)]
     ow, oh = pkg.output_size(w, h, x0, y0, levels, top)
     check_out = 0 if (ncomp == 1 and fmt != pkg.FMT_NONE) else -1
     desc = pkg.tile_desc(coef_ptrs=coef, coef_pitch=w, width=w, height=h, x0=x0, y0=y0, levels=levels,
                          method=pkg.W5X3 if filt == 53 else pkg.W9X7, ncomp=ncomp, out_format=fmt,
-                         bit_depth=12 if fmt == pkg.FMT_48BITS_RGB else 8,
+                         bit_depth=bit_depth or (12 if fmt == pkg.FMT_48BITS_RGB else 8),
                          check_out=check_out, r53_mode=r53, out_ptr=0x9 << 36, out_pitch_px=ow,
                          planes_ptrs=planes if fmt == pkg.FMT_NONE else None, planes_pitch=ow,
                          top_level=top)
@@ -373,3 +373,12 @@ def test_mixed_batch_addresses(pkg, workloads, mode):
             strip, segi = local % J.nstrips, local // J.nstrips
             (item_two_level if J.fuse2 else item_one_level)(bufs, J, strip, segi, 3)
     assert not bufs.bad, bufs.bad[:3]
+
+
+def test_random_case_addresses(pkg):
+    """The cases of the GPU random sweep, checked here first."""
+    from random_cases import random_cases
+    for c in random_cases():
+        w, h, levels, x0, y0, filt, ncomp, fmt, bd, r53, seg, fuse = c
+        bad = check_tile(pkg, w, h, levels, x0, y0, filt, ncomp, fmt, r53, seg, fuse, bit_depth=bd)
+        assert not bad, (c, bad[:3])

@@ -48,9 +48,10 @@ def test_53_line_matches_t800_all_sizes(orc, n):
 @pytest.mark.parametrize("levels", [1, 2, 3, 5, 6])
 def test_53_textbook_reversible(orc, levels):
     """forward 5/3 (T.800 F.4) then inverse is the identity, any size and
-    parity (T.800 F.3 reversibility)."""
-    for (w, h) in [(1, 1), (2, 3), (7, 5), (33, 17), (64, 64), (97, 131)]:
-        for (x0, y0) in [(0, 0), (1, 0), (0, 3), (5, 7)]:
+    parity (T.800 F.3 reversibility); the small sizes at odd offsets have
+    empty coarse levels (no rows or no columns), which transform nothing."""
+    for (w, h) in [(1, 1), (2, 1), (1, 2), (3, 2), (2, 3), (7, 5), (33, 17), (64, 64), (97, 131)]:
+        for (x0, y0) in [(0, 0), (1, 0), (0, 3), (5, 7), (3, 1)]:
             img = RNG.integers(-2048, 2048, (h, w)).astype(np.int32)
             coef = orc.fdwt53(img, levels, x0=x0, y0=y0)
             back = orc.idwt53(coef, levels, x0=x0, y0=y0, mode=orc.R53_TEXTBOOK)
