@@ -7,8 +7,9 @@ import numpy as np
 
 SIDES = [1, 2, 3, 5, 8, 13, 21, 34, 55, 89, 144, 233, 300]
 # (ncomp, out_format, bit_depth): 7 = no packing (planes), 2 = 24-bit RGB,
-# 1 = 48-bit RGB, 6 = 24-bit merge (one component as grey)
-OUTPUTS = [(3, 2, 8), (3, 1, 12), (1, 7, 8), (3, 7, 8), (1, 6, 8)]
+# 1 = 48-bit RGB, 6 = 24-bit merge (one component as grey), 0 = 32-bit
+# ARGB, 3 = 32-bit merge of four components
+OUTPUTS = [(3, 2, 8), (3, 1, 12), (1, 7, 8), (3, 7, 8), (1, 6, 8), (3, 0, 8), (4, 3, 8)]
 
 
 def random_cases(n=120, seed=11):

@@ -1,6 +1,6 @@
 """Seeded random tiles (tests/random_cases.py) decoded by the fused engine
 and compared with the oracle bit for bit: planes (9/7 as float bits),
-24-bit and 48-bit RGB and merged grey, at every offset parity, 1..5
+24-bit and 48-bit RGB, 32-bit ARGB, merged grey and four-component merges, at every offset parity, 1..5
 levels, both 5/3 flavours, several segment heights, the two-level step on
 and off.  Every case's kernel addresses are checked on the CPU first
 (test_kernel_addresses.py::test_random_case_addresses)."""
